@@ -1,0 +1,142 @@
+"""ctypes mirror of include/gpk_c.h (structs, enums, flags).
+
+Kept in one place so the product bindings and the test-side checkers agree
+on the POD layouts that cross the C ABI.
+"""
+import ctypes as C
+import math
+
+# status codes
+GPK_OK = 0
+GPK_E_INVALID = -1
+GPK_E_HIP = -2
+GPK_E_UNSUPPORTED_MODEL = -3
+GPK_E_LABEL_RANGE = -4
+GPK_E_OOM = -5
+GPK_E_NO_DEVICE = -6
+GPK_E_STALLED = -7
+
+# precision modes
+STRICT32 = 0
+FP32 = 1
+FP64 = 2
+PRECISIONS = {"strict32": STRICT32, "fp32": FP32, "fp64": FP64}
+
+# fields: (id, kind) with kind "i32", "u8" or "real"
+FIELDS = {
+    "phi": (0, "i32"), "alpha": (1, "real"), "c": (2, "real"), "T": (3, "real"),
+    "g": (4, "real"), "rho": (5, "real"), "nx": (6, "real"), "ny": (7, "real"),
+    "nz": (8, "real"), "kappa": (9, "real"), "vn": (10, "real"),
+    "phi_comp": (11, "i32"), "g_comp": (12, "real"), "mask": (13, "u8"),
+}
+
+# model kinds
+ISOTROPIC = 0
+ANISO_GROWTH = 1
+SOLIDIFICATION = 2
+TENSOR_ANISO = 3
+ANISO_OSTWALD = 4
+
+FLAG_GRAIN_NETWORK = 0x1
+FLAG_SOLUTE_REJECTION = 0x2
+FLAG_STOCHASTIC = 0x4
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("flags", C.c_uint32),
+        ("sigma0", C.c_float), ("eps4", C.c_float), ("eps6", C.c_float),
+        ("M0", C.c_float), ("k_partition", C.c_float), ("noise_scale", C.c_float),
+        ("kB", C.c_float),
+        ("m_liq", C.c_float), ("k_part", C.c_float), ("T_melt", C.c_float), ("L", C.c_float),
+        ("bxx", C.c_float), ("byy", C.c_float), ("bzz", C.c_float),
+        ("c_sat", C.c_float), ("Omega", C.c_float), ("dG_bias", C.c_float),
+    ]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("Nz", C.c_int32), ("Ny", C.c_int32), ("Nx", C.c_int32),
+        ("dx", C.c_float), ("D", C.c_float), ("gaussian_sigma", C.c_float),
+        ("precision", C.c_int32), ("device", C.c_int32),
+    ]
+
+
+class StepStats(C.Structure):
+    _fields_ = [
+        ("band_cells", C.c_int64), ("candidates", C.c_int64), ("accepted", C.c_int64),
+        ("deferred", C.c_int64), ("exchanges", C.c_int64), ("eroded", C.c_int64),
+        ("dt", C.c_double), ("time", C.c_double), ("step_id", C.c_int64),
+    ]
+
+
+def model_defaults(kind):
+    """Defaults of the reference ThermoModel base (include/gpk_core.hpp:73-123)
+    and of each model's constructor (include/gpk_thermo.hpp)."""
+    m = ModelDesc()
+    m.kind = kind
+    m.flags = 0
+    m.sigma0 = 0.5
+    m.eps4 = 0.0
+    m.eps6 = 0.0
+    m.M0 = 1e-8
+    m.k_partition = 1.0
+    m.noise_scale = 1.0
+    m.kB = 1.380649e-23
+    m.m_liq, m.k_part, m.T_melt, m.L = -500.0, 0.3, 933.0, 1e8
+    m.bxx = m.byy = m.bzz = 1.0
+    m.c_sat, m.Omega, m.dG_bias = 0.0, 0.0, 0.0
+    if kind in (ISOTROPIC, ANISO_GROWTH):
+        m.flags = FLAG_GRAIN_NETWORK
+    if kind == SOLIDIFICATION:
+        m.M0, m.sigma0, m.eps4 = 5e-8, 0.2, 0.04
+    return m
+
+
+def isotropic(M0, sigma0=0.5):
+    """IsotropicGrowthThermo(M0, sigma0) (include/gpk_thermo.hpp:18-27)."""
+    m = model_defaults(ISOTROPIC)
+    m.M0, m.sigma0, m.eps4 = M0, sigma0, 0.0
+    return m
+
+
+def anisotropic_growth(M0, sigma0, eps4, eps6=0.0):
+    """AnisotropicGrowthThermo(M0, sigma0, eps4) (include/gpk_thermo.hpp:33-45);
+    eps6 is a 2-D six-fold extension (0 reproduces the reference law)."""
+    m = model_defaults(ANISO_GROWTH)
+    m.M0, m.sigma0, m.eps4, m.eps6 = M0, sigma0, eps4, eps6
+    return m
+
+
+def solidification(M0=5e-8, sigma0=0.2, eps4=0.04, m_liq=-500.0, k_part=0.3,
+                   T_melt=933.0, L=1e8, solute_rejection=False):
+    """SolidificationThermo (include/gpk_thermo.hpp:52-85); solute_rejection
+    sets ThermoModel::solute_rejection with k_partition = k_part, as the
+    solidification driver does (demos/gpk.cpp:329-330)."""
+    m = model_defaults(SOLIDIFICATION)
+    m.M0, m.sigma0, m.eps4 = M0, sigma0, eps4
+    m.m_liq, m.k_part, m.T_melt, m.L = m_liq, k_part, T_melt, L
+    if solute_rejection:
+        m.flags |= FLAG_SOLUTE_REJECTION
+        m.k_partition = k_part
+    return m
+
+
+def aniso_ostwald(M0, sigma0, bxx, byy, bzz, c_sat, Omega):
+    """AnisoOstwaldThermo (include/gpk_thermo.hpp:284-325)."""
+    m = model_defaults(ANISO_OSTWALD)
+    m.M0, m.sigma0 = M0, sigma0
+    m.bxx, m.byy, m.bzz, m.c_sat, m.Omega = bxx, byy, bzz, c_sat, Omega
+    return m
+
+
+def tensor_aniso(M0, sigma0, bxx=1.0, byy=1.0, bzz=1.0, dG_bias=0.0):
+    """TensorAnisotropyThermo (include/gpk_thermo.hpp:229-263)."""
+    m = model_defaults(TENSOR_ANISO)
+    m.M0, m.sigma0 = M0, sigma0
+    m.bxx, m.byy, m.bzz, m.dG_bias = bxx, byy, bzz, dG_bias
+    return m
+
+
+def half_pi_f32():
+    return float(C.c_float(math.pi * 0.5).value)

@@ -1,0 +1,1336 @@
+// gpk_hip.hip — host orchestration and C ABI of libgpk_hip.so.
+//
+// One handle = one grid on one device with one HIP stream.  The per-step
+// pipeline (GPKGrid::step, src/gpk_simulation.cpp:31-54) runs as a sequence
+// of hand-written kernels on that stream; the host reads back only the few
+// counts that size the next launch (band size, candidates, proposals) and
+// the undecided count of the ordered transactions.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <atomic>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <queue>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <type_traits>
+#include <unordered_set>
+#include <vector>
+
+#include "gpk_kernels.hpp"
+
+using namespace gpkd;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct GpkError : std::runtime_error {
+    int code;
+    GpkError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess)                                                             \
+            throw GpkError(GPK_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+inline int nblk(long long n, int b) { return int((n + b - 1) / b); }
+inline int gsblk(long long n, int b) { return std::max(1, std::min(nblk(n, b), 8192)); }
+
+// ---------------------------------------------------------------------------
+// Device buffer helper
+// ---------------------------------------------------------------------------
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void alloc(size_t b) {
+        release();
+        if (b == 0) b = 16;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw GpkError(GPK_E_OOM, "hipMalloc(" + std::to_string(b) + ") failed");
+        }
+        bytes = b;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    ~DevBuf() { release(); }
+};
+
+// ---------------------------------------------------------------------------
+// Host-side weight construction (include/gpk_core.hpp:37-55,
+// src/gpk_geometry.cpp:12-36).  The float build uses the same float
+// expressions as the reference so the tables are bit-identical.
+// ---------------------------------------------------------------------------
+struct Weights {
+    int radius = 0;        // Gaussian radius
+    int rg = 0;            // geometry stencil radius
+    float sigma_cells = 0.f;
+    std::vector<double> w0, w1, w2;   // stored as double, exact for float values
+};
+
+Weights gauss_float(float sigma) {
+    Weights W;
+    W.sigma_cells = std::max(sigma, 0.5f);
+    const float s = W.sigma_cells;
+    W.radius = static_cast<int>(std::ceil(3.5f * s));
+    const int n = 2 * W.radius + 1;
+    std::vector<float> w(n);
+    float sum = 0.f;
+    for (int i = 0; i < n; ++i) {
+        const float x = float(i - W.radius);
+        w[i] = std::exp(-x * x / (2.f * s * s));
+        sum += w[i];
+    }
+    for (auto& v : w) v /= sum;
+    W.rg = W.radius + 2;
+    const int width = 2 * W.rg + 1;
+    std::vector<float> a(width, 0.f), b(width, 0.f), c(width, 0.f);
+    const int o1[4] = {-2, -1, 1, 2};
+    const float c1[4] = {1.f / 12.f, -8.f / 12.f, 8.f / 12.f, -1.f / 12.f};
+    const int o2[5] = {-2, -1, 0, 1, 2};
+    const float c2[5] = {-1.f / 12.f, 16.f / 12.f, -30.f / 12.f, 16.f / 12.f, -1.f / 12.f};
+    auto at = [&](int o) { return (o < -W.radius || o > W.radius) ? 0.f : w[o + W.radius]; };
+    for (int o = -W.rg; o <= W.rg; ++o) {
+        const int j = o + W.rg;
+        a[j] = at(o);
+        for (int k = 0; k < 4; ++k) b[j] += c1[k] * at(o - o1[k]);
+        for (int k = 0; k < 5; ++k) c[j] += c2[k] * at(o - o2[k]);
+    }
+    W.w0.assign(a.begin(), a.end());
+    W.w1.assign(b.begin(), b.end());
+    W.w2.assign(c.begin(), c.end());
+    return W;
+}
+
+Weights gauss_double(float sigma) {
+    Weights W;
+    W.sigma_cells = std::max(sigma, 0.5f);
+    const double s = W.sigma_cells;
+    W.radius = static_cast<int>(std::ceil(3.5f * W.sigma_cells));
+    const int n = 2 * W.radius + 1;
+    std::vector<double> w(n);
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double x = double(i - W.radius);
+        w[i] = std::exp(-x * x / (2.0 * s * s));
+        sum += w[i];
+    }
+    for (auto& v : w) v /= sum;
+    W.rg = W.radius + 2;
+    const int width = 2 * W.rg + 1;
+    W.w0.assign(width, 0.0); W.w1.assign(width, 0.0); W.w2.assign(width, 0.0);
+    const int o1[4] = {-2, -1, 1, 2};
+    const double c1[4] = {1.0 / 12.0, -8.0 / 12.0, 8.0 / 12.0, -1.0 / 12.0};
+    const int o2[5] = {-2, -1, 0, 1, 2};
+    const double c2[5] = {-1.0 / 12.0, 16.0 / 12.0, -30.0 / 12.0, 16.0 / 12.0, -1.0 / 12.0};
+    auto at = [&](int o) { return (o < -W.radius || o > W.radius) ? 0.0 : w[o + W.radius]; };
+    for (int o = -W.rg; o <= W.rg; ++o) {
+        const int j = o + W.rg;
+        W.w0[j] = at(o);
+        for (int k = 0; k < 4; ++k) W.w1[j] += c1[k] * at(o - o1[k]);
+        for (int k = 0; k < 5; ++k) W.w2[j] += c2[k] * at(o - o2[k]);
+    }
+    return W;
+}
+
+// exp(-r^2/(2 s^2)) indexed by integer r^2 (src/gpk_physics.cpp:160-185,
+// src/gpk_transitions.cpp:50-81): expf promoted to double in float modes.
+std::vector<double> radial_table(float s, int r, bool single) {
+    const int n = 3 * r * r + 1;
+    std::vector<double> t(n);
+    if (single) {
+        const float inv = 0.5f / (s * s);
+        for (int r2 = 0; r2 < n; ++r2) {
+            const float r2f = static_cast<float>(r2);
+            t[r2] = static_cast<double>(std::exp(-r2f * inv));
+        }
+    } else {
+        const double sd = s;
+        const double inv = 0.5 / (sd * sd);
+        for (int r2 = 0; r2 < n; ++r2) t[r2] = std::exp(-double(r2) * inv);
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// Voronoi generator (demos/gpk.cpp:51-189).  Site draws use std::mt19937 and
+// the libstdc++ distributions; the nearest-site labels come from k_voronoi;
+// connectivity normalisation runs on the host over the label array.
+// ---------------------------------------------------------------------------
+struct Sites {
+    std::vector<int> z, y, x;
+    std::vector<float> g;
+};
+
+Sites draw_sites(int Nz, int Ny, int Nx, int n, uint32_t seed) {
+    Sites S;
+    std::mt19937 rng(seed);
+    std::uniform_int_distribution<int> rz(0, Nz - 1), ry(0, Ny - 1), rx(0, Nx - 1);
+    std::uniform_real_distribution<float> rg(0.f, float(M_PI) * 0.5f);
+    S.z.resize(n); S.y.resize(n); S.x.resize(n); S.g.resize(n);
+    std::unordered_set<long long> used;
+    used.reserve(size_t(2) * n);
+    for (int k = 0; k < n; ++k) {
+        long long lin;
+        do {
+            S.z[k] = rz(rng); S.y[k] = ry(rng); S.x[k] = rx(rng);
+            lin = (static_cast<long long>(S.z[k]) * Ny + S.y[k]) * Nx + S.x[k];
+        } while (!used.insert(lin).second);
+        S.g[k] = rg(rng);
+    }
+    return S;
+}
+
+// host nearest-site search over a bucket grid; exact (all sites at the
+// minimal distance are inspected, lowest index wins)
+void nearest_host(int Nz, int Ny, int Nx, const Sites& S, int32_t* phi) {
+    const int n = static_cast<int>(S.z.size());
+    const long long N = 1LL * Nz * Ny * Nx;
+    const double per = std::max(1.0, double(N) / n);
+    int B = std::max(1, static_cast<int>(std::cbrt(per)));
+    if (Nz == 1) B = std::max(1, static_cast<int>(std::sqrt(per)));
+    const int bz = Nz == 1 ? 1 : std::max(1, (Nz + B - 1) / B);
+    const int by = std::max(1, (Ny + B - 1) / B), bx = std::max(1, (Nx + B - 1) / B);
+    const int Bz = Nz == 1 ? 1 : B;
+    std::vector<std::vector<int>> bucket(size_t(bz) * by * bx);
+    for (int k = 0; k < n; ++k)
+        bucket[(size_t(S.z[k] / Bz) * by + S.y[k] / B) * bx + S.x[k] / B].push_back(k);
+    auto worker = [&](int z0, int z1) {
+        for (int z = z0; z < z1; ++z)
+            for (int y = 0; y < Ny; ++y)
+                for (int x = 0; x < Nx; ++x) {
+                    long long bd = LLONG_MAX;
+                    int best = 0;
+                    const int cz = z / Bz, cy = y / B, cx = x / B;
+                    for (int ring = 0;; ++ring) {
+                        // ring of buckets at Chebyshev bucket distance `ring`
+                        const int rz = Nz == 1 ? 0 : ring;
+                        for (int dz = -rz; dz <= rz; ++dz)
+                            for (int dy = -ring; dy <= ring; ++dy)
+                                for (int dxb = -ring; dxb <= ring; ++dxb) {
+                                    if (std::max({std::abs(dz), std::abs(dy), std::abs(dxb)}) != ring) continue;
+                                    if (2 * std::abs(dz) + 1 > bz + 1 && bz > 1 && std::abs(dz) * 2 > bz) continue;
+                                    const int qz = ((cz + dz) % bz + bz) % bz;
+                                    const int qy = ((cy + dy) % by + by) % by;
+                                    const int qx = ((cx + dxb) % bx + bx) % bx;
+                                    for (int k : bucket[(size_t(qz) * by + qy) * bx + qx]) {
+                                        long long a = std::abs(z - S.z[k]), b = std::abs(y - S.y[k]),
+                                                  e = std::abs(x - S.x[k]);
+                                        a = std::min(a, (long long)Nz - a);
+                                        b = std::min(b, (long long)Ny - b);
+                                        e = std::min(e, (long long)Nx - e);
+                                        const long long d = a * a + b * b + e * e;
+                                        if (d < bd || (d == bd && k < best)) { bd = d; best = k; }
+                                    }
+                                }
+                        // every site outside rings 0..ring lies at a periodic
+                        // distance of at least ring*B cells along some axis
+                        const long long reach = 1LL * ring * B;
+                        const bool covered_all = (2 * ring + 1 >= bx) && (2 * ring + 1 >= by) &&
+                                                 (Nz == 1 || 2 * ring + 1 >= bz);
+                        if (covered_all) break;
+                        if (bd != LLONG_MAX && reach * reach > bd) break;
+                    }
+                    phi[(size_t(z) * Ny + y) * Nx + x] = best + 1;
+                }
+    };
+    const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int step = (Nz + nt - 1) / nt;
+    if (Nz >= nt) {
+        for (int t = 0; t < nt; ++t) th.emplace_back(worker, t * step, std::min(Nz, (t + 1) * step));
+        for (auto& t : th) t.join();
+    } else {
+        worker(0, Nz);
+    }
+}
+
+// keep the largest face component of each label; attach the rest layer by layer
+int normalise_components(int Nz, int Ny, int Nx, int n_seeds, const Sites& S, int32_t* phi,
+                         float* g) {
+    const int N = Nz * Ny * Nx;
+    for (int i = 0; i < N; ++i) g[i] = S.g[phi[i] - 1];
+    auto idx = [&](int z, int y, int x) { return (z * Ny + y) * Nx + x; };
+    auto nbrs = [&](int c, int o[6]) {
+        const int x = c % Nx, y = (c / Nx) % Ny, z = c / (Ny * Nx);
+        const int xm = x == 0 ? Nx - 1 : x - 1, xp = x == Nx - 1 ? 0 : x + 1;
+        const int ym = y == 0 ? Ny - 1 : y - 1, yp = y == Ny - 1 ? 0 : y + 1;
+        const int zm = z == 0 ? Nz - 1 : z - 1, zp = z == Nz - 1 ? 0 : z + 1;
+        o[0] = idx(z, y, xm); o[1] = idx(z, y, xp); o[2] = idx(z, ym, x);
+        o[3] = idx(z, yp, x); o[4] = idx(zm, y, x); o[5] = idx(zp, y, x);
+    };
+    const int nface = Nz > 1 ? 6 : 4;
+    std::vector<int> comp(N, -1);
+    std::vector<long long> csize;
+    std::vector<int32_t> clabel;
+    std::vector<int> largest(size_t(n_seeds) + 1, -1);
+    std::vector<int> queue;
+    queue.reserve(1024);
+    for (int s = 0; s < N; ++s) {
+        if (comp[s] >= 0) continue;
+        const int id = static_cast<int>(csize.size());
+        const int32_t lab = phi[s];
+        queue.clear();
+        queue.push_back(s);
+        comp[s] = id;
+        for (size_t h = 0; h < queue.size(); ++h) {
+            int f[6];
+            nbrs(queue[h], f);
+            for (int k = 0; k < nface; ++k)
+                if (comp[f[k]] < 0 && phi[f[k]] == lab) { comp[f[k]] = id; queue.push_back(f[k]); }
+        }
+        csize.push_back(static_cast<long long>(queue.size()));
+        clabel.push_back(lab);
+        const int prev = largest[lab];
+        if (prev < 0 || csize[id] > csize[prev]) largest[lab] = id;
+    }
+    std::vector<uint8_t> att(N, 0);
+    long long natt = 0;
+    for (int i = 0; i < N; ++i)
+        if (largest[clabel[comp[i]]] == comp[i]) { att[i] = 1; ++natt; }
+    std::vector<int> cells;
+    std::vector<int32_t> tg;
+    std::vector<float> og;
+    while (natt < N) {
+        cells.clear(); tg.clear(); og.clear();
+        for (int cc = 0; cc < N; ++cc) {
+            if (att[cc]) continue;
+            int f[6];
+            nbrs(cc, f);
+            int32_t cand[6];
+            int sup[6] = {0, 0, 0, 0, 0, 0}, src[6], nc = 0;
+            for (int k = 0; k < nface; ++k) {
+                if (!att[f[k]]) continue;
+                const int32_t v = phi[f[k]];
+                int q = 0;
+                while (q < nc && cand[q] != v) ++q;
+                if (q == nc) { cand[q] = v; src[q] = f[k]; ++nc; }
+                ++sup[q];
+            }
+            if (nc == 0) continue;
+            int b = 0;
+            for (int q = 1; q < nc; ++q)
+                if (sup[q] > sup[b] || (sup[q] == sup[b] && cand[q] < cand[b])) b = q;
+            cells.push_back(cc); tg.push_back(cand[b]); og.push_back(g[src[b]]);
+        }
+        if (cells.empty()) return GPK_E_STALLED;
+        for (size_t k = 0; k < cells.size(); ++k) {
+            phi[cells[k]] = tg[k]; g[cells[k]] = og[k]; att[cells[k]] = 1;
+        }
+        natt += static_cast<long long>(cells.size());
+    }
+    return GPK_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Engine interface
+// ---------------------------------------------------------------------------
+struct EngineBase {
+    virtual ~EngineBase() = default;
+    virtual void upload(int field, const void* src, size_t bytes, bool device) = 0;
+    virtual void download(int field, void* dst, size_t bytes, bool device) = 0;
+    virtual void step(float dt, float cap, gpk_step_stats* st) = 0;
+    virtual void set_model(const gpk_model_desc& m) = 0;
+    virtual double time() = 0;
+    virtual void set_time(double t) = 0;
+    virtual double max_dt() = 0;
+    virtual double phase_volume(int32_t l) = 0;
+    virtual int n_grains() = 0;
+    virtual int interface_cells() = 0;
+    virtual double mean_grain_radius() = 0;
+    virtual float solid_fraction() = 0;
+    virtual float interface_fraction() = 0;
+    virtual double solute_sum() = 0;
+    virtual size_t device_bytes() const = 0;
+    virtual int simple_move(int cell, int32_t nl) = 0;
+    virtual void voronoi(int n_seeds, uint32_t seed) = 0;
+    virtual void sync() = 0;
+    virtual void set_profiling(int on) = 0;
+    virtual void timing(int which, double* ms, int64_t* n, double* bytes) = 0;
+    virtual int64_t steps() const = 0;
+};
+
+template <class R, bool kDirect>
+struct Engine final : EngineBase {
+    // ---- configuration ----
+    int Nz, Ny, Nx, N, dim3;
+    float dx_f, D_f, sigma_f;
+    gpk_model_desc model;
+    int device;
+    hipStream_t stream = nullptr;
+    Dims dims;
+    Weights W;
+    int proj_r = 1, remap_r = 1;
+    float vel_sigma = 0.85f, remap_sigma = 0.7f;
+    int64_t step_id = 0;
+    int nlab = 0;           // label table size (max label + 1)
+
+    // ---- device state ----
+    DevBuf phi, mask, phi_comp, slot, cand_id, pid, mark;
+    DevBuf alpha, c, T, g, rho, g_comp, work1, work2, work3, work4;
+    DevBuf bandA, bandB, nx, ny, nz, kappa, vraw, vn;
+    DevBuf cflag, anew_s, sel;
+    DevBuf Ccell, Cold, Cnew, Cg, Canew, state, pos_old, pos_new, qflags, queue0, queue1, fbq;
+    DevBuf Pk, Ps;
+    DevBuf gain_s, pcell, kgain, kgain2, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
+        pflag2, pidx;
+    DevBuf Pa, Pb, Plo, Phi, pstate, Pg;
+    DevBuf count, unres, ulist, utgt, utgg, uflag;
+    DevBuf wt0, wt1, wt2, tab, projw, remapw;
+    DevBuf scal, dsum, tagctr, minmax, noise_amp;
+    DevBuf cub_tmp;
+    size_t cub_bytes = 0;
+    Scalars* hs = nullptr;      // pinned host mirror of the scalar block
+    int geo_band = 0;           // size of the geometry band (bandA)
+    bool fields_valid = false;
+
+    // profiling
+    bool profiling = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double geo_ms = 0.0;
+    int64_t geo_launches = 0;
+    double geo_bytes = 0.0;
+
+    Engine(const gpk_grid_desc& gd, const gpk_model_desc& m)
+        : Nz(gd.Nz), Ny(gd.Ny), Nx(gd.Nx), N(gd.Nz * gd.Ny * gd.Nx), dim3(gd.Nz > 1 ? 1 : 0),
+          dx_f(gd.dx), D_f(gd.D), sigma_f(gd.gaussian_sigma), model(m), device(gd.device) {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        dims = Dims{Nz, Ny, Nx, N};
+        W = (sizeof(R) == 4) ? gauss_float(sigma_f) : gauss_double(sigma_f);
+        vel_sigma = std::max(Nz > 1 ? std::clamp(0.375f * sigma_f, 0.85f, 3.00f) : 0.85f, 0.5f);
+        remap_sigma = std::max(Nz > 1 ? std::clamp(0.125f * sigma_f, 0.70f, 1.00f) : 0.70f, 0.5f);
+        proj_r = std::max(1, static_cast<int>(std::ceil(3.f * vel_sigma)));
+        remap_r = std::max(1, static_cast<int>(std::ceil(3.f * remap_sigma)));
+        allocate();
+        HIPCHK(hipEventCreate(&ev0));
+        HIPCHK(hipEventCreate(&ev1));
+    }
+
+    ~Engine() override {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (hs) (void)hipHostFree(hs);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    int64_t steps() const override { return step_id; }
+
+    size_t device_bytes() const override {
+        size_t t = 0;
+        const DevBuf* all[] = {&phi, &mask, &phi_comp, &slot, &cand_id, &pid, &mark, &alpha, &c, &T,
+                               &g, &rho, &g_comp, &work1, &work2, &work3, &work4, &bandA, &bandB,
+                               &nx, &ny, &nz, &kappa, &vraw, &vn, &cflag, &anew_s, &sel, &Ccell,
+                               &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
+                               &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain, &kgain2,
+                               &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
+                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &count, &unres, &ulist,
+                               &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
+                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp};
+        for (auto* b : all) t += b->bytes;
+        return t;
+    }
+
+    template <class T>
+    void fill(DevBuf& b, long long n, T v) {
+        if (sizeof(T) == 4 && std::is_integral<T>::value)
+            k_fill_i32<<<gsblk(n, 256), 256, 0, stream>>>(int(n), b.as<int32_t>(), int32_t(v));
+        else if (sizeof(T) == 1)
+            k_fill_u8<<<gsblk(n, 256), 256, 0, stream>>>(int(n), b.as<uint8_t>(), uint8_t(v));
+        else
+            k_fill_r<T><<<gsblk(n, 256), 256, 0, stream>>>(int(n), b.as<T>(), v);
+    }
+
+    void allocate() {
+        const size_t n = size_t(N), rs = sizeof(R);
+        phi.alloc(n * 4); mask.alloc(n); phi_comp.alloc(n * 4); slot.alloc(n * 4);
+        cand_id.alloc(n * 4); pid.alloc(n * 4); mark.alloc(n * 4);
+        alpha.alloc(n * rs); c.alloc(n * rs); T.alloc(n * rs); g.alloc(n * rs); rho.alloc(n * rs);
+        g_comp.alloc(n * rs); work1.alloc(n * rs); work2.alloc(n * rs); work3.alloc(n * rs);
+        work4.alloc(n * rs);
+        bandA.alloc(n * 4); bandB.alloc(n * 4);
+        nx.alloc(n * rs); ny.alloc(n * rs); nz.alloc(n * rs); kappa.alloc(n * rs);
+        vraw.alloc(n * rs); vn.alloc(n * rs);
+        cflag.alloc(n); anew_s.alloc(n * rs); sel.alloc(n * 4);
+        Ccell.alloc(n * 4); Cold.alloc(n * 4); Cnew.alloc(n * 4); Cg.alloc(n * rs);
+        Canew.alloc(n * rs); state.alloc(n * 4); pos_old.alloc(n * 4); pos_new.alloc(n * 4);
+        qflags.alloc(n * 4); queue0.alloc(n * 4); queue1.alloc(n * 4); fbq.alloc(n * 4);
+        Pk.alloc(2 * n * 8); Ps.alloc(2 * n * 8);
+        gain_s.alloc(n * rs); pcell.alloc(n * 4); kgain.alloc(n * 8); kgain2.alloc(n * 8);
+        idx.alloc(n * 4); order.alloc(n * 4); kgrp.alloc(n * 8); kgrp2.alloc(n * 8);
+        order2.alloc(n * 4); seg_start.alloc(n * 4); seg_end.alloc(n * 4); pflag2.alloc(n * 4);
+        pidx.alloc(n * 4);
+        Pa.alloc(n * 4); Pb.alloc(n * 4); Plo.alloc(n * 4); Phi.alloc(n * 4); pstate.alloc(n * 4);
+        Pg.alloc(2 * n * rs);
+        ulist.alloc(n * 4); utgt.alloc(n * 4); utgg.alloc(n * rs); uflag.alloc(n);
+        scal.alloc(sizeof(Scalars)); dsum.alloc(((n + 1023) / 1024 + 16) * 8);
+        tagctr.alloc(16); minmax.alloc(16); noise_amp.alloc(16);
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(Scalars), hipHostMallocDefault));
+        std::memset(hs, 0, sizeof(Scalars));
+        // weights
+        std::vector<R> w0(W.w0.begin(), W.w0.end()), w1(W.w1.begin(), W.w1.end()),
+            w2(W.w2.begin(), W.w2.end());
+        const int Wd = 2 * W.rg + 1;
+        wt0.alloc(Wd * rs); wt1.alloc(Wd * rs); wt2.alloc(Wd * rs);
+        HIPCHK(hipMemcpy(wt0.p, w0.data(), Wd * rs, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(wt1.p, w1.data(), Wd * rs, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(wt2.p, w2.data(), Wd * rs, hipMemcpyHostToDevice));
+        // direct-stencil product table, (gx*gy)*gz in the reference's order
+        if (kDirect) {
+            std::vector<R> t;
+            if (dim3) {
+                t.resize(size_t(Wd) * Wd * Wd * 9);
+                size_t o = 0;
+                for (int jz = 0; jz < Wd; ++jz)
+                    for (int jy = 0; jy < Wd; ++jy)
+                        for (int jx = 0; jx < Wd; ++jx) {
+                            const R gx0 = w0[jx], gx1 = w1[jx], gx2 = w2[jx];
+                            const R gy0 = w0[jy], gy1 = w1[jy], gy2 = w2[jy];
+                            const R gz0 = w0[jz], gz1 = w1[jz], gz2 = w2[jz];
+                            t[o++] = gx1 * gy0 * gz0; t[o++] = gx0 * gy1 * gz0;
+                            t[o++] = gx2 * gy0 * gz0; t[o++] = gx0 * gy2 * gz0;
+                            t[o++] = gx1 * gy1 * gz0; t[o++] = gx0 * gy0 * gz1;
+                            t[o++] = gx0 * gy0 * gz2; t[o++] = gx1 * gy0 * gz1;
+                            t[o++] = gx0 * gy1 * gz1;
+                        }
+            } else {
+                t.resize(size_t(Wd) * Wd * 5);
+                size_t o = 0;
+                const R one = R(1);
+                for (int jy = 0; jy < Wd; ++jy)
+                    for (int jx = 0; jx < Wd; ++jx) {
+                        const R gx0 = w0[jx], gx1 = w1[jx], gx2 = w2[jx];
+                        const R gy0 = w0[jy], gy1 = w1[jy], gy2 = w2[jy];
+                        t[o++] = gx1 * gy0 * one; t[o++] = gx0 * gy1 * one;
+                        t[o++] = gx2 * gy0 * one; t[o++] = gx0 * gy2 * one;
+                        t[o++] = gx1 * gy1 * one;
+                    }
+            }
+            tab.alloc(t.size() * rs);
+            HIPCHK(hipMemcpy(tab.p, t.data(), t.size() * rs, hipMemcpyHostToDevice));
+        } else {
+            tab.alloc(16);
+        }
+        const bool single = sizeof(R) == 4;
+        const auto pw = radial_table(vel_sigma, proj_r, single);
+        const auto rw = radial_table(remap_sigma, remap_r, single);
+        projw.alloc(pw.size() * 8); remapw.alloc(rw.size() * 8);
+        HIPCHK(hipMemcpy(projw.p, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(remapw.p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
+        // hipcub scratch: sized for the largest call
+        size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
+        hipcub::CountingInputIterator<int> it(0);
+        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b1, it, mask.as<uint8_t>(), sel.as<int>(),
+                                             &scal.as<Scalars>()->nsel, N, stream));
+        HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, Pk.as<unsigned long long>(),
+                                                 Ps.as<unsigned long long>(), 2 * N, 0, 64, stream));
+        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, kgain.as<unsigned long long>(),
+                                                  kgain2.as<unsigned long long>(), idx.as<int>(),
+                                                  order.as<int>(), N, 0, 64, stream));
+        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b4, pflag2.as<int>(), pidx.as<int>(), N,
+                                                stream));
+        cub_bytes = std::max({b1, b2, b3, b4}) + 256;
+        cub_tmp.alloc(cub_bytes);
+        // initial values (src/gpk_grid.cpp:14-21)
+        fill<int32_t>(phi, N, 0);
+        fill<uint8_t>(mask, N, 0);
+        fill<int32_t>(phi_comp, N, 0);
+        fill<int32_t>(cand_id, N, -1);
+        fill<int32_t>(pid, N, -1);
+        fill<int32_t>(mark, N, 0);
+        fill<R>(alpha, N, R(1));
+        fill<R>(c, N, R(0));
+        fill<R>(T, N, R(800));
+        fill<R>(g, N, R(0));
+        fill<R>(rho, N, R(0));
+        fill<R>(g_comp, N, R(0));
+        HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(Scalars), stream));
+        HIPCHK(hipMemsetAsync(tagctr.p, 0, 16, stream));
+        nlab = 1;
+        count.alloc(size_t(nlab + 1) * 4);
+        unres.alloc(size_t(nlab + 1));
+        rebuild_counts();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // ---- scalar helpers ----
+    Scalars* ds() { return scal.as<Scalars>(); }
+    void pull_scalars() {
+        HIPCHK(hipMemcpyAsync(hs, scal.p, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void set_int(int* dev, int v) {
+        k_fill_i32<<<1, 1, 0, stream>>>(1, dev, v);
+    }
+
+    void rebuild_counts() {
+        fill<int32_t>(count, nlab, 0);
+        k_label_hist<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), count.as<int>());
+    }
+
+    void check_labels() {
+        int init[2] = {INT_MAX, INT_MIN};
+        HIPCHK(hipMemcpyAsync(minmax.p, init, 8, hipMemcpyHostToDevice, stream));
+        k_minmax_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), minmax.as<int>());
+        int mm[2];
+        HIPCHK(hipMemcpyAsync(mm, minmax.p, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (mm[0] < 0 || mm[1] >= GPK_MAX_LABEL)
+            throw GpkError(GPK_E_LABEL_RANGE, "labels must lie in [0, 2^26); found [" +
+                                                  std::to_string(mm[0]) + ", " + std::to_string(mm[1]) + "]");
+        const int need = mm[1] + 1;
+        if (need > nlab || count.bytes < size_t(need) * 4) {
+            nlab = need;
+            count.alloc(size_t(nlab) * 4);
+            unres.alloc(size_t(nlab));
+        }
+        nlab = std::max(nlab, need);
+        rebuild_counts();
+    }
+
+    // ---- field access ----
+    DevBuf* field_buf(int f, int& kind) {
+        kind = 2;   // real
+        switch (f) {
+        case GPK_FIELD_PHI: kind = 0; return &phi;
+        case GPK_FIELD_PHI_COMP: kind = 0; return &phi_comp;
+        case GPK_FIELD_MASK: kind = 1; return &mask;
+        case GPK_FIELD_ALPHA: return &alpha;
+        case GPK_FIELD_C: return &c;
+        case GPK_FIELD_T: return &T;
+        case GPK_FIELD_G: return &g;
+        case GPK_FIELD_RHO: return &rho;
+        case GPK_FIELD_G_COMP: return &g_comp;
+        case GPK_FIELD_NX: kind = 3; return &nx;
+        case GPK_FIELD_NY: kind = 3; return &ny;
+        case GPK_FIELD_NZ: kind = 3; return &nz;
+        case GPK_FIELD_KAPPA: kind = 3; return &kappa;
+        case GPK_FIELD_VN: kind = 3; return &vn;
+        default: return nullptr;
+        }
+    }
+    size_t elem(int kind) const { return kind == 0 ? 4 : kind == 1 ? 1 : sizeof(R); }
+
+    void upload(int f, const void* src, size_t bytes, bool device) override {
+        int kind;
+        DevBuf* b = field_buf(f, kind);
+        if (!b || kind == 3) throw GpkError(GPK_E_INVALID, "field is not writable");
+        if (!src || bytes != size_t(N) * elem(kind)) throw GpkError(GPK_E_INVALID, "upload size mismatch");
+        HIPCHK(hipMemcpyAsync(b->p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                              stream));
+        if (f == GPK_FIELD_PHI) check_labels();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void download(int f, void* dst, size_t bytes, bool device) override {
+        int kind;
+        DevBuf* b = field_buf(f, kind);
+        if (!b) throw GpkError(GPK_E_INVALID, "unknown field");
+        if (!dst || bytes != size_t(N) * elem(kind)) throw GpkError(GPK_E_INVALID, "download size mismatch");
+        const hipMemcpyKind mk = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+        if (kind == 3) {
+            // band-compact field: scatter into zeros over the full grid
+            fill<R>(work4, N, R(0));
+            if (geo_band > 0)
+                k_scatter_band<R><<<nblk(geo_band, 256), 256, 0, stream>>>(geo_band, bandA.as<int>(),
+                                                                           b->as<R>(), work4.as<R>());
+            HIPCHK(hipMemcpyAsync(dst, work4.p, bytes, mk, stream));
+        } else {
+            HIPCHK(hipMemcpyAsync(dst, b->p, bytes, mk, stream));
+        }
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void set_model(const gpk_model_desc& m) override {
+        if (m.kind != model.kind) throw GpkError(GPK_E_INVALID, "model kind cannot change");
+        model = m;
+    }
+    double time() override {
+        pull_scalars();
+        return hs->time;
+    }
+    void set_time(double t) override {
+        HIPCHK(hipMemcpyAsync(&ds()->time, &t, 8, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void sync() override { HIPCHK(hipStreamSynchronize(stream)); }
+    void set_profiling(int on) override {
+        profiling = on != 0;
+        geo_ms = 0.0;
+        geo_launches = 0;
+    }
+    void timing(int which, double* ms, int64_t* n, double* bytes) override {
+        (void)which;
+        if (ms) *ms = geo_launches ? geo_ms / double(geo_launches) : 0.0;
+        if (n) *n = geo_launches;
+        if (bytes) *bytes = geo_launches ? geo_bytes / double(geo_launches) : 0.0;
+    }
+
+    // ---- band compaction: mask + sorted interface list (+ slot map) ----
+    int band_compact(DevBuf& list, bool with_slot) {
+        k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
+        hipcub::CountingInputIterator<int> it(0);
+        size_t tb = cub_bytes;
+        HIPCHK(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, mask.as<uint8_t>(), list.as<int>(),
+                                             &ds()->nsel, N, stream));
+        pull_scalars();
+        const int nb = hs->nsel;
+        if (with_slot && nb > 0)
+            k_slot<<<nblk(nb, 256), 256, 0, stream>>>(nb, list.as<int>(), slot.as<int>());
+        return nb;
+    }
+
+    int select_flags(const uint8_t* flags, int n, int* out) {
+        if (n <= 0) return 0;
+        hipcub::CountingInputIterator<int> it(0);
+        size_t tb = cub_bytes;
+        HIPCHK(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, flags, out, &ds()->nsel, n, stream));
+        pull_scalars();
+        return hs->nsel;
+    }
+
+    // ---- the step (src/gpk_simulation.cpp:31-54) ----
+    void step(float dt_req, float cap, gpk_step_stats* st) override {
+        const int nb = band_compact(bandA, true);
+        geo_band = nb;
+        // geometry + competitor + constitutive velocity
+        GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
+        if (nb > 0) {
+            if (profiling) HIPCHK(hipEventRecord(ev0, stream));
+            if (dim3)
+                k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
+                    wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                    nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+            else
+                k_geometry<R, 2, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
+                    wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                    nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+            if (profiling) {
+                HIPCHK(hipEventRecord(ev1, stream));
+                HIPCHK(hipEventSynchronize(ev1));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
+                geo_ms += ms;
+                ++geo_launches;
+                // compulsory state traffic of the step per launch: labels of the
+                // grid (4 B/cell) + alpha read/write and g read per band cell
+                geo_bytes += 4.0 * N + (2.0 * sizeof(R) + sizeof(R)) * nb;
+            }
+        }
+        k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
+        if (nb > 0) {
+            if (dim3)
+                k_project<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(),
+                    phi.as<int32_t>(), phi_comp.as<int32_t>(), projw.as<double>(), vraw.as<R>(),
+                    vn.as<R>(), reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+            else
+                k_project<R, 2><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(),
+                    phi.as<int32_t>(), phi_comp.as<int32_t>(), projw.as<double>(), vraw.as<R>(),
+                    vn.as<R>(), reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+        }
+        k_dt<R><<<1, 1, 0, stream>>>(ds(), R(dt_req), R(cap), dx_f, D_f, dim3);
+        const bool partition = (model.flags & GPK_FLAG_SOLUTE_REJECTION) != 0;
+        if (partition)
+            k_solid_fraction<R><<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), alpha.as<R>(),
+                                                                   work1.as<R>());
+        // counters
+        set_int(&ds()->accepted, 0);
+        set_int(&ds()->deferred, 0);
+        set_int(&ds()->exchanges, 0);
+        set_int(&ds()->eroded, 0);
+        int m = 0;
+        if (nb > 0) m = flips(nb);
+        pull_scalars();
+        const int accepted = hs->accepted;
+        int exchanges = 0;
+        if (accepted > 0) {
+            exchanges = remap_sweep();
+            if (dim3 && exchanges > 0) exchanges += remap_sweep();
+        }
+        if ((model.flags & GPK_FLAG_GRAIN_NETWORK) && (uint64_t(step_id) & 7ULL) == 0ULL) extinction();
+        if (partition) {
+            transport_partitioned();
+        } else if (D_f > 0.f) {
+            k_diffusion<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, c.as<R>(), mask.as<uint8_t>(),
+                                                              alpha.as<R>(), ds(), dx_f, D_f, work2.as<R>());
+            std::swap(c, work2);
+        }
+        k_time_advance<<<1, 1, 0, stream>>>(ds());
+        ++step_id;
+        if (st) {
+            pull_scalars();
+            st->band_cells = nb;
+            st->candidates = m;
+            st->accepted = hs->accepted;
+            st->deferred = hs->deferred;
+            st->exchanges = hs->exchanges;
+            st->eroded = hs->eroded;
+            st->dt = hs->dt;
+            st->time = hs->time;
+            st->step_id = step_id;
+        }
+    }
+
+    // ---- alpha integration + ordered flip transaction ----
+    int flips(int nb) {
+        const bool noise = (model.flags & GPK_FLAG_STOCHASTIC) != 0;
+        float amp = 0.f;
+        if (noise) {
+            k_noise_amp<R><<<1, 1, 0, stream>>>(ds(), dx_f, model.M0, model.kB, model.noise_scale,
+                                                noise_amp.as<float>());
+            HIPCHK(hipMemcpyAsync(&amp, noise_amp.p, 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        k_alpha<R><<<nblk(nb, 256), 256, 0, stream>>>(nb, bandA.as<int>(), ds(), dx_f, vn.as<R>(),
+                                                      alpha.as<R>(), T.as<R>(), phi.as<int32_t>(),
+                                                      phi_comp.as<int32_t>(), cflag.as<uint8_t>(),
+                                                      anew_s.as<R>(), noise ? 1 : 0, amp,
+                                                      uint64_t(step_id));
+        const int m = select_flags(cflag.as<uint8_t>(), nb, sel.as<int>());
+        if (m == 0) return 0;
+        const bool network = (model.flags & GPK_FLAG_GRAIN_NETWORK) != 0;
+        k_cand_init<R><<<nblk(m, 256), 256, 0, stream>>>(
+            m, sel.as<int>(), bandA.as<int>(), phi.as<int32_t>(), phi_comp.as<int32_t>(), g_comp.as<R>(),
+            anew_s.as<R>(), Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(),
+            state.as<int>(), cand_id.as<int>(), network ? Pk.as<unsigned long long>() : nullptr);
+        if (network) {
+            size_t tb = cub_bytes;
+            int hibit = 32;
+            while ((1LL << (hibit - 32)) < nlab) ++hibit;
+            HIPCHK(hipcub::DeviceRadixSort::SortKeys(cub_tmp.p, tb, Pk.as<unsigned long long>(),
+                                                     Ps.as<unsigned long long>(), 2 * m, 0,
+                                                     std::min(64, hibit + 1), stream));
+            k_cand_pos<<<nblk(2 * m, 256), 256, 0, stream>>>(2 * m, Ps.as<unsigned long long>(),
+                                                              Cold.as<int>(), pos_old.as<int>(),
+                                                              pos_new.as<int>());
+            set_int(&ds()->undecided, m);
+            set_int(&ds()->qcount[0], 0);
+            set_int(&ds()->qcount[1], 0);
+            set_int(&ds()->fbcount, 0);
+            FlipCtx fc;
+            fc.d = dims; fc.dim3 = dim3; fc.m = m;
+            fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
+            fc.state = state.as<int>(); fc.cand_id = cand_id.as<int>(); fc.phi = phi.as<int32_t>();
+            fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>();
+            fc.pos_new = pos_new.as<int>(); fc.count = count.as<int>();
+            fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
+            fc.qflags = qflags.as<int>(); fc.s = ds();
+            int round = 0;
+            const int bfs_grid = std::max(1, std::min(m, 512));
+            while (true) {
+                for (int r = 0; r < 4; ++r, ++round) {
+                    k_flip_round<<<nblk(m, 256), 256, 0, stream>>>(fc, round);
+                    k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, round, fbq.as<int>());
+                    k_flip_bfs_big<<<1, 1024, 0, stream>>>(fc, fbq.as<int>(), mark.as<int>(),
+                                                           reinterpret_cast<int*>(work3.p),
+                                                           reinterpret_cast<int*>(work4.p), tagctr.as<int>());
+                }
+                pull_scalars();
+                if (hs->undecided <= 0) break;
+                if (round > 4 * (m + 8))
+                    throw GpkError(GPK_E_HIP, "flip transaction did not converge");
+            }
+        }
+        k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
+            m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
+            phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
+            network ? 0 : 1);
+        return m;
+    }
+
+    // ---- balanced Gaussian remap sweep (src/gpk_transitions.cpp:38-178) ----
+    int remap_sweep() {
+        set_int(&ds()->exchanges, 0);
+        const int nb = band_compact(bandB, false);
+        k_copy_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), phi_comp.as<int32_t>());
+        if (nb > 0) {
+            if (dim3)
+                k_remap_score<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, remap_r, nb, bandB.as<int>(), phi.as<int32_t>(), remapw.as<double>(),
+                    phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+            else
+                k_remap_score<R, 2><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, remap_r, nb, bandB.as<int>(), phi.as<int32_t>(), remapw.as<double>(),
+                    phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+        }
+        const int np = select_flags(cflag.as<uint8_t>(), nb, sel.as<int>());
+        int npairs = 0;
+        if (np > 0) {
+            k_prop_keys<R><<<nblk(np, 256), 256, 0, stream>>>(np, sel.as<int>(), bandB.as<int>(),
+                                                              gain_s.as<R>(), phi.as<int32_t>(),
+                                                              phi_comp.as<int32_t>(), pcell.as<int>(),
+                                                              kgain.as<unsigned long long>(), idx.as<int>());
+            size_t tb = cub_bytes;
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp.p, tb, kgain.as<unsigned long long>(),
+                                                      kgain2.as<unsigned long long>(), idx.as<int>(),
+                                                      order.as<int>(), np, 0, sizeof(R) == 4 ? 32 : 64,
+                                                      stream));
+            k_prop_group_keys<<<nblk(np, 256), 256, 0, stream>>>(np, order.as<int>(), pcell.as<int>(),
+                                                                  phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                                                                  kgrp.as<unsigned long long>());
+            tb = cub_bytes;
+            HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp.p, tb, kgrp.as<unsigned long long>(),
+                                                      kgrp2.as<unsigned long long>(), order.as<int>(),
+                                                      order2.as<int>(), np, 0, 64, stream));
+            k_seg_bounds<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
+                                                             seg_start.as<int>(), seg_end.as<int>());
+            k_pair_flags<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
+                                                             seg_start.as<int>(), seg_end.as<int>(),
+                                                             pflag2.as<int>());
+            tb = cub_bytes;
+            HIPCHK(hipcub::DeviceScan::ExclusiveSum(cub_tmp.p, tb, pflag2.as<int>(), pidx.as<int>(), np,
+                                                    stream));
+            int tail[2];
+            HIPCHK(hipMemcpyAsync(&tail[0], pidx.as<int>() + (np - 1), 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipMemcpyAsync(&tail[1], pflag2.as<int>() + (np - 1), 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            npairs = tail[0] + tail[1];
+            if (npairs > 0) {
+                k_pair_emit<R><<<nblk(np, 256), 256, 0, stream>>>(
+                    np, kgrp2.as<unsigned long long>(), seg_start.as<int>(), seg_end.as<int>(),
+                    pflag2.as<int>(), pidx.as<int>(), order2.as<int>(), pcell.as<int>(), Pa.as<int>(),
+                    Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(), pid.as<int>());
+                set_int(&ds()->undecided, npairs);
+                PairCtx pc;
+                pc.d = dims; pc.dim3 = dim3; pc.np = npairs;
+                pc.Pa = Pa.as<int>(); pc.Pb = Pb.as<int>(); pc.Plo = Plo.as<int>(); pc.Phi = Phi.as<int>();
+                pc.pstate = pstate.as<int>(); pc.pid = pid.as<int>(); pc.phi = phi.as<int32_t>();
+                pc.g = g.p; pc.Pg = Pg.p; pc.count = count.as<int>(); pc.s = ds();
+                int rounds = 0;
+                while (true) {
+                    for (int r = 0; r < 4; ++r, ++rounds)
+                        k_pair_round<R><<<nblk(npairs, 256), 256, 0, stream>>>(pc);
+                    pull_scalars();
+                    if (hs->undecided <= 0) break;
+                    if (rounds > 4 * (npairs + 8))
+                        throw GpkError(GPK_E_HIP, "remap transaction did not converge");
+                }
+                k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
+                    npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
+                    Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds());
+            }
+        }
+        band_compact(bandB, false);
+        pull_scalars();
+        return hs->exchanges;
+    }
+
+    // ---- unresolved-grain extinction (src/gpk_transitions.cpp:180-249) ----
+    void extinction() {
+        const double s = std::max(W.sigma_cells, 1.f);
+        const double vol = dim3 ? (4.0 * M_PI * s * s * s / 3.0) : (M_PI * s * s);
+        const int min_cells = std::max(2, static_cast<int>(std::ceil(vol)));
+        set_int(&ds()->present, 0);
+        set_int(&ds()->unresolved, 0);
+        k_label_census<<<nblk(nlab, 256), 256, 0, stream>>>(nlab, count.as<int>(), min_cells,
+                                                            unres.as<uint8_t>(), ds());
+        pull_scalars();
+        if (hs->unresolved == 0 || hs->unresolved == hs->present) return;
+        k_unres_flags<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), unres.as<uint8_t>(),
+                                                         uflag.as<uint8_t>());
+        const int n = select_flags(uflag.as<uint8_t>(), N, ulist.as<int>());
+        if (n > 0)
+            k_erode<R><<<1, 1024, 0, stream>>>(dims, dim3, n, ulist.as<int>(), unres.as<uint8_t>(),
+                                               phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(),
+                                               utgt.as<int>(), utgg.as<R>(), ds());
+        band_compact(bandB, false);
+    }
+
+    // ---- partitioned solute transport (src/gpk_transport.cpp:48-135) ----
+    void transport_partitioned() {
+        const float k = model.k_partition;
+        if (k < 1.f) {
+            // work1 holds the solid fraction before the flip pass
+            k_reject_source<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, k, phi.as<int32_t>(),
+                                                                  alpha.as<R>(), work1.as<R>(), c.as<R>(),
+                                                                  work2.as<R>(), work3.as<R>());
+            k_reject_gather<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, phi.as<int32_t>(),
+                                                                  alpha.as<R>(), work2.as<R>(),
+                                                                  work3.as<R>(), c.as<R>());
+        }
+        if (D_f > 0.f) {
+            k_liquid_comp<R><<<gsblk(N, 256), 256, 0, stream>>>(N, k, phi.as<int32_t>(), alpha.as<R>(),
+                                                                c.as<R>(), work1.as<R>(), work2.as<R>());
+            k_diffusion_part<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, work1.as<R>(), work2.as<R>(),
+                                                                   c.as<R>(), ds(), dx_f, D_f, work3.as<R>());
+            std::swap(c, work3);
+        }
+    }
+
+    // ---- diagnostics ----
+    // GPKGrid::max_dt() (src/gpk_simulation.cpp:11-25) reads vn over the
+    // latest compacted band; vn is zero outside the geometry band
+    double max_dt() override {
+        k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
+        if (geo_band > 0)
+            k_vmax_masked<R><<<nblk(geo_band, 256), 256, 0, stream>>>(
+                geo_band, bandA.as<int>(), mask.as<uint8_t>(), vn.as<R>(),
+                reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+        const double saved_dt = 0.0;
+        (void)saved_dt;
+        DevBuf tmp;
+        tmp.alloc(sizeof(Scalars));
+        HIPCHK(hipMemcpyAsync(tmp.p, scal.p, sizeof(Scalars), hipMemcpyDeviceToDevice, stream));
+        k_dt<R><<<1, 1, 0, stream>>>(tmp.as<Scalars>(), R(-1), R(Eps<R>::big), dx_f, D_f, dim3);
+        Scalars out;
+        HIPCHK(hipMemcpyAsync(&out, tmp.p, sizeof(Scalars), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return out.dt;
+    }
+
+    double mean_grain_radius() override {
+        // src/gpk_simulation.cpp:88-106
+        const int ng = n_grains();
+        if (ng == 0) return 0.0;
+        const double mv = double(N) / ng;
+        const double Rr = dim3 ? std::cbrt(3.0 * mv / (4.0 * M_PI)) : std::sqrt(mv / M_PI);
+        return Rr * double(dx_f);
+    }
+    float solid_fraction() override {
+        return static_cast<float>(phase_volume(1) / static_cast<double>(N));
+    }
+    float interface_fraction() override {
+        return static_cast<float>(interface_cells()) / static_cast<float>(N);
+    }
+
+    double phase_volume(int32_t label) override {
+        const int nch = (N + 1023) / 1024;
+        k_phase_terms<R><<<nblk(nch, 256), 256, 0, stream>>>(N, label, phi.as<int32_t>(),
+                                                             phi_comp.as<int32_t>(), mask.as<uint8_t>(),
+                                                             alpha.as<R>(), dsum.as<double>());
+        k_sum_final<<<1, 1, 0, stream>>>(nch, dsum.as<double>(), dsum.as<double>() + nch);
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, dsum.as<double>() + nch, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v;
+    }
+
+    int n_grains() override {
+        set_int(&ds()->present, 0);
+        k_count_pos<<<gsblk(nlab, 256), 256, 0, stream>>>(nlab, count.as<int>(), &ds()->present);
+        pull_scalars();
+        return hs->present;
+    }
+
+    int interface_cells() override {
+        set_int(&ds()->nsel, 0);
+        k_count_u8<<<gsblk(N, 256), 256, 0, stream>>>(N, mask.as<uint8_t>(), &ds()->nsel);
+        pull_scalars();
+        return hs->nsel;
+    }
+
+    double solute_sum() override {
+        const int nch = (N + 1023) / 1024;
+        k_sum_chunks<R><<<nblk(nch, 256), 256, 0, stream>>>(N, c.as<R>(), dsum.as<double>());
+        k_sum_final<<<1, 1, 0, stream>>>(nch, dsum.as<double>(), dsum.as<double>() + nch);
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, dsum.as<double>() + nch, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v;
+    }
+
+    int simple_move(int cell, int32_t nl) override {
+        if (cell < 0 || cell >= N) return 0;
+        // a one-candidate transaction evaluated with the flip machinery
+        int32_t ol = 0;
+        HIPCHK(hipMemcpyAsync(&ol, phi.as<int32_t>() + cell, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        if (ol == nl) return 0;
+        if (nl < 0 || nl >= nlab) {
+            // a label absent from the grid has no face branch: new_object == 0
+            return 0;
+        }
+        int one[3] = {cell, ol, nl};
+        HIPCHK(hipMemcpyAsync(Ccell.p, &one[0], 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(Cold.p, &one[1], 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(Cnew.p, &one[2], 4, hipMemcpyHostToDevice, stream));
+        set_int(state.as<int>(), UNDEC);
+        unsigned long long keys[2] = {((unsigned long long)(uint32_t)ol << 32),
+                                      ((unsigned long long)(uint32_t)nl << 32)};
+        if (ol > nl) std::swap(keys[0], keys[1]);
+        HIPCHK(hipMemcpyAsync(Ps.p, keys, 16, hipMemcpyHostToDevice, stream));
+        int po = ol < nl ? 0 : 1, pn = 1 - po;
+        HIPCHK(hipMemcpyAsync(pos_old.p, &po, 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(pos_new.p, &pn, 4, hipMemcpyHostToDevice, stream));
+        set_int(&ds()->undecided, 1);
+        set_int(&ds()->qcount[0], 0);
+        set_int(&ds()->qcount[1], 0);
+        set_int(&ds()->fbcount, 0);
+        FlipCtx fc;
+        fc.d = dims; fc.dim3 = dim3; fc.m = 1;
+        fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
+        fc.state = state.as<int>(); fc.cand_id = cand_id.as<int>(); fc.phi = phi.as<int32_t>();
+        fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>(); fc.pos_new = pos_new.as<int>();
+        fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
+        fc.qflags = qflags.as<int>(); fc.s = ds();
+        k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
+        k_flip_bfs<<<1, 256, 0, stream>>>(fc, 0, fbq.as<int>());
+        k_flip_bfs_big<<<1, 1024, 0, stream>>>(fc, fbq.as<int>(), mark.as<int>(),
+                                               reinterpret_cast<int*>(work3.p),
+                                               reinterpret_cast<int*>(work4.p), tagctr.as<int>());
+        int st = 0;
+        HIPCHK(hipMemcpyAsync(&st, state.p, 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return st == ACCEPT ? 1 : 0;
+    }
+
+    void voronoi(int n_seeds, uint32_t seed) override {
+        if (n_seeds <= 0 || n_seeds > N)
+            throw GpkError(GPK_E_INVALID, "Voronoi seed count must lie between one and the cell count");
+        Sites S = draw_sites(Nz, Ny, Nx, n_seeds, seed);
+        std::vector<int32_t> h_phi(N);
+        // nearest site on the device
+        DevBuf sz, sy, sx;
+        sz.alloc(size_t(n_seeds) * 4); sy.alloc(size_t(n_seeds) * 4); sx.alloc(size_t(n_seeds) * 4);
+        HIPCHK(hipMemcpyAsync(sz.p, S.z.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(sy.p, S.y.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(sx.p, S.x.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
+        k_voronoi<<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(dims, n_seeds, sz.as<int>(), sy.as<int>(),
+                                                                sx.as<int>(), phi.as<int32_t>());
+        HIPCHK(hipMemcpyAsync(h_phi.data(), phi.p, size_t(N) * 4, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        std::vector<float> h_g(N);
+        const int rc = normalise_components(Nz, Ny, Nx, n_seeds, S, h_phi.data(), h_g.data());
+        if (rc != GPK_OK) throw GpkError(rc, "Voronoi connectivity normalization stalled");
+        std::vector<R> gr(h_g.begin(), h_g.end());
+        HIPCHK(hipMemcpyAsync(phi.p, h_phi.data(), size_t(N) * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(g.p, gr.data(), size_t(N) * sizeof(R), hipMemcpyHostToDevice, stream));
+        fill<R>(alpha, N, R(1));
+        check_labels();
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+};
+
+EngineBase* make_engine(const gpk_grid_desc& gd, const gpk_model_desc& m) {
+    switch (gd.precision) {
+    case GPK_PREC_STRICT32: return new Engine<float, true>(gd, m);
+    case GPK_PREC_FP32: return new Engine<float, false>(gd, m);
+    case GPK_PREC_FP64: return new Engine<double, false>(gd, m);
+    default: throw GpkError(GPK_E_INVALID, "unknown precision mode");
+    }
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        g_last_error.clear();
+        return GPK_OK;
+    } catch (const GpkError& e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return GPK_E_INVALID;
+    } catch (...) {
+        g_last_error = "unknown error";
+        return GPK_E_INVALID;
+    }
+}
+
+} // namespace
+
+struct gpk_handle {
+    std::unique_ptr<EngineBase> eng;
+};
+
+extern "C" {
+
+const char* gpk_last_error(void) { return g_last_error.c_str(); }
+const char* gpk_version(void) { return "gpk_hip 0.1 (gfx950)"; }
+
+void gpk_model_defaults(int32_t kind, gpk_model_desc* m) {
+    if (!m) return;
+    std::memset(m, 0, sizeof(*m));
+    m->kind = kind;
+    m->sigma0 = 0.5f; m->eps4 = 0.f; m->eps6 = 0.f; m->M0 = 1e-8f;
+    m->k_partition = 1.f; m->noise_scale = 1.f; m->kB = 1.380649e-23f;
+    m->m_liq = -500.f; m->k_part = 0.3f; m->T_melt = 933.f; m->L = 1e8f;
+    m->bxx = m->byy = m->bzz = 1.f;
+    if (kind == GPK_MODEL_ISOTROPIC || kind == GPK_MODEL_ANISO_GROWTH) m->flags = GPK_FLAG_GRAIN_NETWORK;
+    if (kind == GPK_MODEL_SOLIDIFICATION) { m->M0 = 5e-8f; m->sigma0 = 0.2f; m->eps4 = 0.04f; }
+}
+
+int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model, gpk_handle** out) {
+    return guarded([&] {
+        if (!grid || !model || !out) throw GpkError(GPK_E_INVALID, "null argument");
+        *out = nullptr;
+        if (grid->Nz <= 0 || grid->Ny <= 0 || grid->Nx <= 0)
+            throw GpkError(GPK_E_INVALID, "grid dimensions must be positive");
+        if (1LL * grid->Nz * grid->Ny * grid->Nx >= (1LL << 31))
+            throw GpkError(GPK_E_INVALID, "grid exceeds 2^31 cells");
+        if (!(grid->dx > 0.f)) throw GpkError(GPK_E_INVALID, "dx must be positive");
+        {
+            const float sc = std::max(grid->gaussian_sigma, 0.5f);
+            const int rg = static_cast<int>(std::ceil(3.5f * sc)) + 2;
+            if (grid->Nz > 1 && rg > 18)
+                throw GpkError(GPK_E_INVALID,
+                               "3-D geometry radius > 18 selects the reference's global separable "
+                               "path, which this library does not implement");
+        }
+        if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_ANISO_OSTWALD)
+            throw GpkError(GPK_E_UNSUPPORTED_MODEL, "unknown model kind");
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+            throw GpkError(GPK_E_NO_DEVICE, "no HIP device visible");
+        if (grid->device < 0 || grid->device >= ndev) throw GpkError(GPK_E_INVALID, "device ordinal out of range");
+        auto h = std::make_unique<gpk_handle>();
+        h->eng.reset(make_engine(*grid, *model));
+        *out = h.release();
+    });
+}
+
+int gpk_destroy(gpk_handle* h) {
+    return guarded([&] { delete h; });
+}
+
+#define NEED_H \
+    if (!h) throw GpkError(GPK_E_INVALID, "null handle")
+
+int gpk_upload(gpk_handle* h, int field, const void* host, size_t bytes) {
+    return guarded([&] { NEED_H; h->eng->upload(field, host, bytes, false); });
+}
+int gpk_download(gpk_handle* h, int field, void* host, size_t bytes) {
+    return guarded([&] { NEED_H; h->eng->download(field, host, bytes, false); });
+}
+int gpk_upload_device(gpk_handle* h, int field, const void* p, size_t bytes) {
+    return guarded([&] { NEED_H; h->eng->upload(field, p, bytes, true); });
+}
+int gpk_download_device(gpk_handle* h, int field, void* p, size_t bytes) {
+    return guarded([&] { NEED_H; h->eng->download(field, p, bytes, true); });
+}
+
+int gpk_step(gpk_handle* h, float dt, float dt_cap, double* dt_used, gpk_step_stats* stats) {
+    return guarded([&] {
+        NEED_H;
+        gpk_step_stats st{};
+        h->eng->step(dt, dt_cap, &st);
+        if (dt_used) *dt_used = st.dt;
+        if (stats) *stats = st;
+    });
+}
+
+int gpk_step_many(gpk_handle* h, int nsteps, float dt, float dt_cap, gpk_step_stats* stats) {
+    return guarded([&] {
+        NEED_H;
+        if (nsteps < 0) throw GpkError(GPK_E_INVALID, "nsteps must be non-negative");
+        gpk_step_stats acc{}, st{};
+        for (int s = 0; s < nsteps; ++s) {
+            h->eng->step(dt, dt_cap, &st);
+            acc.band_cells += st.band_cells; acc.candidates += st.candidates;
+            acc.accepted += st.accepted; acc.deferred += st.deferred;
+            acc.exchanges += st.exchanges; acc.eroded += st.eroded;
+            acc.dt = st.dt; acc.time = st.time; acc.step_id = st.step_id;
+        }
+        if (stats) *stats = acc;
+    });
+}
+
+int gpk_run(gpk_handle* h, double t_end, float dt, int* steps_taken) {
+    return guarded([&] {
+        NEED_H;
+        int n = 0;
+        double t = h->eng->time();
+        while (t < t_end) {
+            const float remaining = static_cast<float>(t_end - t);
+            gpk_step_stats st{};
+            h->eng->step(dt, remaining, &st);
+            t = st.time;
+            ++n;
+        }
+        if (steps_taken) *steps_taken = n;
+    });
+}
+
+int gpk_set_model(gpk_handle* h, const gpk_model_desc* m) {
+    return guarded([&] {
+        NEED_H;
+        if (!m) throw GpkError(GPK_E_INVALID, "null model");
+        h->eng->set_model(*m);
+    });
+}
+
+int gpk_get_time(gpk_handle* h, double* t) {
+    return guarded([&] { NEED_H; if (!t) throw GpkError(GPK_E_INVALID, "null"); *t = h->eng->time(); });
+}
+int gpk_set_time(gpk_handle* h, double t) {
+    return guarded([&] { NEED_H; h->eng->set_time(t); });
+}
+int gpk_max_dt(gpk_handle* h, double* dt) {
+    return guarded([&] { NEED_H; if (!dt) throw GpkError(GPK_E_INVALID, "null"); *dt = h->eng->max_dt(); });
+}
+int gpk_phase_volume(gpk_handle* h, int32_t label, double* v) {
+    return guarded([&] { NEED_H; if (!v) throw GpkError(GPK_E_INVALID, "null"); *v = h->eng->phase_volume(label); });
+}
+int gpk_n_grains(gpk_handle* h, int* n) {
+    return guarded([&] { NEED_H; if (!n) throw GpkError(GPK_E_INVALID, "null"); *n = h->eng->n_grains(); });
+}
+int gpk_mean_grain_radius(gpk_handle* h, double* r) {
+    return guarded([&] { NEED_H; if (!r) throw GpkError(GPK_E_INVALID, "null"); *r = h->eng->mean_grain_radius(); });
+}
+int gpk_solid_fraction(gpk_handle* h, float* f) {
+    return guarded([&] { NEED_H; if (!f) throw GpkError(GPK_E_INVALID, "null"); *f = h->eng->solid_fraction(); });
+}
+int gpk_interface_fraction(gpk_handle* h, float* f) {
+    return guarded([&] { NEED_H; if (!f) throw GpkError(GPK_E_INVALID, "null"); *f = h->eng->interface_fraction(); });
+}
+int gpk_solute_sum(gpk_handle* h, double* s) {
+    return guarded([&] { NEED_H; if (!s) throw GpkError(GPK_E_INVALID, "null"); *s = h->eng->solute_sum(); });
+}
+int gpk_device_bytes(gpk_handle* h, size_t* b) {
+    return guarded([&] { NEED_H; if (!b) throw GpkError(GPK_E_INVALID, "null"); *b = h->eng->device_bytes(); });
+}
+int gpk_topology_simple_move(gpk_handle* h, int cell, int32_t new_label, int* result) {
+    return guarded([&] {
+        NEED_H;
+        if (!result) throw GpkError(GPK_E_INVALID, "null");
+        *result = h->eng->simple_move(cell, new_label);
+    });
+}
+int gpk_voronoi_init(gpk_handle* h, int n_seeds, uint32_t seed) {
+    return guarded([&] { NEED_H; h->eng->voronoi(n_seeds, seed); });
+}
+int gpk_voronoi_host(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, float* g) {
+    return guarded([&] {
+        if (!phi || !g || Nz <= 0 || Ny <= 0 || Nx <= 0) throw GpkError(GPK_E_INVALID, "bad argument");
+        const long long N = 1LL * Nz * Ny * Nx;
+        if (n_seeds <= 0 || n_seeds > N)
+            throw GpkError(GPK_E_INVALID, "Voronoi seed count must lie between one and the cell count");
+        Sites S = draw_sites(Nz, Ny, Nx, n_seeds, seed);
+        nearest_host(Nz, Ny, Nx, S, phi);
+        const int rc = normalise_components(Nz, Ny, Nx, n_seeds, S, phi, g);
+        if (rc != GPK_OK) throw GpkError(rc, "Voronoi connectivity normalization stalled");
+    });
+}
+int gpk_sync(gpk_handle* h) {
+    return guarded([&] { NEED_H; h->eng->sync(); });
+}
+int gpk_set_profiling(gpk_handle* h, int enable) {
+    return guarded([&] { NEED_H; h->eng->set_profiling(enable); });
+}
+int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launches, double* bytes) {
+    return guarded([&] { NEED_H; h->eng->timing(which, avg_ms, launches, bytes); });
+}
+
+} // extern "C"

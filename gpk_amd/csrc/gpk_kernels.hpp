@@ -1,0 +1,1576 @@
+// gpk_kernels.hpp — HIP kernels of the GPK per-step kinetics (gfx950).
+//
+// Data layout in HBM: full-grid SoA fields (phi int32, alpha/c/T/g/g_comp
+// real, mask uint8, phi_comp int32) plus band-compact arrays indexed by the
+// band slot a (sorted interface list `band[a]`, inverse map `slot[cell]`):
+// normals, curvature, raw and projected velocity.  All reductions that feed
+// a decision are exact (integer, max) or sequential per output element, so
+// every kernel is deterministic and reproduces the CPU restatement's
+// operation order.
+#pragma once
+#include "gpk_device.hpp"
+
+namespace gpkd {
+
+enum : int { UNDEC = 0, ACCEPT = 1, REJECT = 2, QUEUED = 3 };
+
+__device__ __forceinline__ int ld_acq(const int* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_rel(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---------------------------------------------------------------------------
+// Interface mask (src/gpk_geometry.cpp:42-82): mask[i] = any of the six
+// periodic face neighbours carries a different label.
+// ---------------------------------------------------------------------------
+__global__ void k_mask(Dims d, const int32_t* __restrict__ phi, uint8_t* __restrict__ mask) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        int f[6];
+        faces(d, i, f);
+        const int32_t v = phi[i];
+        mask[i] = uint8_t((v != phi[f[0]]) | (v != phi[f[1]]) | (v != phi[f[2]]) |
+                          (v != phi[f[3]]) | (v != phi[f[4]]) | (v != phi[f[5]]));
+    }
+}
+
+__global__ void k_slot(int n, const int* __restrict__ band, int* __restrict__ slot) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < n) slot[band[a]] = a;
+}
+
+// ---------------------------------------------------------------------------
+// Geometry (src/gpk_geometry.cpp:88-183) fused with the competitor choice and
+// the constitutive velocity (src/gpk_physics.cpp:67-148).
+//   kDirect: the reference's masked tensor stencil, accumulated in dz,dy,dx
+//            order from the host-built product table tab[t*9+c] = (gx*gy)*gz
+//            (bit-identical to forming the products inline).
+//   else   : separable masked row sums over x, FMA over y then z.
+// ---------------------------------------------------------------------------
+struct GeomParams {
+    int rg;                 // stencil radius
+    int W;                  // 2*rg+1
+    float dx_f;
+    float sigma_cells;
+};
+
+template <class R> struct Eps;
+template <> struct Eps<float> {
+    static constexpr float grad = 1e-8f, kappa = 1e-6f, flip = 1e-6f, gain = 1e-7f;
+    static constexpr float vmax0 = 1e-30f, big = 1e30f, c_iface = 0.125f, c_diff = 0.4f;
+    static constexpr float cl_floor = 1e-6f;
+};
+template <> struct Eps<double> {
+    static constexpr double grad = 1e-8, kappa = 1e-6, flip = 1e-6, gain = 1e-7;
+    static constexpr double vmax0 = 1e-30, big = 1e30, c_iface = 0.125, c_diff = 0.4;
+    static constexpr double cl_floor = 1e-6;
+};
+
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <class R, int DIM, bool kDirect>
+__global__ void __launch_bounds__(256)
+k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
+           const int* __restrict__ band, const int32_t* __restrict__ phi,
+           const R* __restrict__ w0, const R* __restrict__ w1, const R* __restrict__ w2,
+           const R* __restrict__ tab,
+           const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+           R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
+           R* __restrict__ okappa, R* __restrict__ vraw,
+           int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nband) return;
+    const int i = band[a];
+    const int32_t res = phi[i];
+    int z0, y0, x0;
+    decode(d, i, z0, y0, x0);
+    const int r = gp.rg, W = gp.W;
+    R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
+    if (kDirect) {
+        int t = 0;
+        const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+        for (int dz = dzb; dz <= dze; ++dz) {
+            const int z = wrapi(z0 + dz, d.Nz);
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = wrapi(y0 + dy, d.Ny);
+                const int rowbase = (z * d.Ny + y) * d.Nx;
+                for (int ddx = -r; ddx <= r; ++ddx, ++t) {
+                    const int x = wrapi(x0 + ddx, d.Nx);
+                    if (phi[rowbase + x] != res) continue;
+                    if (DIM == 3) {
+                        const R* p = tab + t * 9;
+                        Ix += p[0]; Iy += p[1]; Ixx += p[2]; Iyy += p[3]; Ixy += p[4];
+                        Iz += p[5]; Izz += p[6]; Ixz += p[7]; Iyz += p[8];
+                    } else {
+                        const R* p = tab + t * 5;
+                        Ix += p[0]; Iy += p[1]; Ixx += p[2]; Iyy += p[3]; Ixy += p[4];
+                    }
+                }
+            }
+        }
+    } else {
+        const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+        for (int dz = dzb; dz <= dze; ++dz) {
+            const int z = wrapi(z0 + dz, d.Nz);
+            R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = wrapi(y0 + dy, d.Ny);
+                const int rowbase = (z * d.Ny + y) * d.Nx;
+                const int jy = dy + r;
+                R R0 = 0, R1 = 0, R2 = 0;
+                for (int ddx = -r; ddx <= r; ++ddx) {
+                    const int x = wrapi(x0 + ddx, d.Nx);
+                    if (phi[rowbase + x] != res) continue;
+                    const int jx = ddx + r;
+                    R0 += w0[jx]; R1 += w1[jx]; R2 += w2[jx];
+                }
+                S00 = fmar(w0[jy], R0, S00); S01 = fmar(w0[jy], R1, S01);
+                S02 = fmar(w0[jy], R2, S02); S10 = fmar(w1[jy], R0, S10);
+                S11 = fmar(w1[jy], R1, S11); S20 = fmar(w2[jy], R0, S20);
+            }
+            if (DIM == 3) {
+                const int jz = dz + r;
+                const R gz0 = w0[jz], gz1 = w1[jz], gz2 = w2[jz];
+                Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
+                Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
+                Ixy = fmar(gz0, S11, Ixy);
+                Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
+                Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
+            } else {
+                Ix = S01; Iy = S10; Ixx = S02; Iyy = S20; Ixy = S11;
+            }
+        }
+    }
+    (void)W;
+    const R dx = R(gp.dx_f);
+    const R inv_dx = R(1) / dx;
+    const R inv_dx2 = inv_dx * inv_dx;
+    const R klim = R(1) / (R(gp.sigma_cells) * dx);
+    Ix *= inv_dx; Iy *= inv_dx; Iz *= inv_dx;
+    Ixx *= inv_dx2; Iyy *= inv_dx2; Izz *= inv_dx2;
+    Ixy *= inv_dx2; Ixz *= inv_dx2; Iyz *= inv_dx2;
+    const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
+    R nxv = 0, nyv = 0, nzv = 0, kap = 0;
+    if (gsq * dx * dx > Eps<R>::grad) {
+        const R gr = dsqrt(gsq);
+        nxv = -Ix / gr; nyv = -Iy / gr; nzv = -Iz / gr;
+        const R trH = Ixx + Iyy + Izz;
+        const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
+                    + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
+        R kv = (gHg - gsq * trH) / (gsq * gr);
+        if (dfabs(kv * dx) < Eps<R>::kappa) kv = R(0);
+        kap = kv < -klim ? -klim : (klim < kv ? klim : kv);
+    }
+    onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
+
+    // competitor: most-supported unlike face label, ties by outward-normal
+    // alignment, then first face (src/gpk_geometry.cpp:154-182)
+    int f[6];
+    faces(d, i, f);
+    const R score[6] = {-nxv, nxv, -nyv, nyv, -nzv, nzv};
+    const int nface = DIM == 3 ? 6 : 4;
+    int32_t cube[27];
+    {
+        int t = 0;
+        const int zb = DIM == 3 ? -1 : 0, ze = DIM == 3 ? 1 : 0;
+        for (int dz = zb; dz <= ze; ++dz) {
+            int zz = z0 + dz; if (zz < 0) zz += d.Nz; else if (zz >= d.Nz) zz -= d.Nz;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = y0 + dy; if (yy < 0) yy += d.Ny; else if (yy >= d.Ny) yy -= d.Ny;
+                for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
+                    int xx = x0 + ddx; if (xx < 0) xx += d.Nx; else if (xx >= d.Nx) xx -= d.Nx;
+                    cube[t] = phi[lin(d, zz, yy, xx)];
+                }
+            }
+        }
+    }
+    const int ncube = DIM == 3 ? 27 : 9, ccentre = DIM == 3 ? 13 : 4;
+    int best = -1, best_support = -1;
+    R best_score = R(-2);
+    for (int k = 0; k < nface; ++k) {
+        const int32_t q = phi[f[k]];
+        if (q == res) continue;
+        int support = 0;
+        for (int t = 0; t < ncube; ++t) support += (t != ccentre) && (cube[t] == q);
+        if (support > best_support || (support == best_support && score[k] > best_score)) {
+            best = k; best_support = support; best_score = score[k];
+        }
+    }
+    const int32_t qid = best >= 0 ? phi[f[best]] : res;
+    phi_comp[i] = qid;
+
+    // constitutive velocity v = M (dG - sigma(n) kappa) (src/gpk_physics.cpp:97-146)
+    const R gpv = g[i];
+    R gq = gpv, cq = c[i];
+    for (int k = 0; k < 6; ++k)
+        if (phi[f[k]] == qid) { gq = g[f[k]]; cq = c[f[k]]; break; }
+    g_comp[i] = gq;
+    const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
+    const R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
+    const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
+    const R mob = R(model.M0);
+    vraw[a] = mob * (dG - sig * kap);
+}
+
+// ---------------------------------------------------------------------------
+// Pair projection (src/gpk_physics.cpp:152-193) + |v| max for the timestep
+// (src/gpk_simulation.cpp:11-25).  Same-pair, orientation-signed Gaussian
+// average over the (2r+1)^d window, FP64 accumulation in window order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void atomic_max_abs(unsigned int* p, float v) {
+    atomicMax(p, __float_as_uint(fabsf(v)));
+}
+__device__ __forceinline__ void atomic_max_abs(unsigned long long* p, double v) {
+    atomicMax(p, (unsigned long long)__double_as_longlong(fabs(v)));
+}
+template <class R> struct UBits;
+template <> struct UBits<float> { using T = unsigned int; };
+template <> struct UBits<double> { using T = unsigned long long; };
+
+template <class R, int DIM>
+__global__ void __launch_bounds__(256)
+k_project(Dims d, int r, int nband, const int* __restrict__ band, const int* __restrict__ slot,
+          const uint8_t* __restrict__ mask, const int32_t* __restrict__ phi,
+          const int32_t* __restrict__ phi_comp, const double* __restrict__ wtab,
+          const R* __restrict__ vraw, R* __restrict__ vout,
+          typename UBits<R>::T* __restrict__ vmax_bits) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nband) return;
+    const int i = band[a];
+    const int32_t p = phi[i], q = phi_comp[i];
+    R out = vraw[a];
+    if (p != q) {
+        const int32_t lo = p < q ? p : q, hi = p < q ? q : p;
+        const R oi = p == lo ? R(1) : R(-1);
+        int z0, y0, x0;
+        decode(d, i, z0, y0, x0);
+        double num = 0.0, den = 0.0;
+        const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+        for (int dz = dzb; dz <= dze; ++dz) {
+            const int z = wrapi(z0 + dz, d.Nz);
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = wrapi(y0 + dy, d.Ny);
+                for (int ddx = -r; ddx <= r; ++ddx) {
+                    const int x = wrapi(x0 + ddx, d.Nx);
+                    const int j = lin(d, z, y, x);
+                    if (!mask[j]) continue;
+                    const int32_t pj = phi[j], qj = phi_comp[j];
+                    if (pj == qj) continue;
+                    const int32_t lj = pj < qj ? pj : qj, hj = pj < qj ? qj : pj;
+                    if (lj != lo || hj != hi) continue;
+                    const double w = wtab[ddx * ddx + dy * dy + dz * dz];
+                    const R oj = pj == lo ? R(1) : R(-1);
+                    num += w * double(oj) * double(vraw[slot[j]]);
+                    den += w;
+                }
+            }
+        }
+        if (den > 0.0) out = oi * R(num / den);
+    }
+    vout[a] = out;
+    atomic_max_abs(vmax_bits, out);
+}
+
+template <class R>
+__global__ void k_vmax_masked(int nband, const int* __restrict__ band, const uint8_t* __restrict__ mask,
+                              const R* __restrict__ vn, typename UBits<R>::T* __restrict__ vmax_bits) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < nband && mask[band[a]]) atomic_max_abs(vmax_bits, vn[a]);
+}
+
+// scalar block shared by the step kernels (device memory)
+struct Scalars {
+    unsigned long long vmax_bits;   // |v| max as bits (float bits in low word for FP32)
+    double dt;                      // timestep of the current step
+    double time;                    // simulation time
+    int band;                       // |A| of the geometry band
+    int band2;                      // size of the latest recomputed band
+    int ncand;                      // flip candidates
+    int undecided;                  // undecided transactions
+    int qcount[2];                  // BFS queues (double-buffered per round)
+    int fbcount;                    // BFS fallback queue
+    int accepted, deferred;
+    int nprop;                      // remap proposals
+    int npairs;
+    int exchanges;
+    int eroded;
+    int layer_changed;
+    int nsel;                       // generic selection count
+    int present, unresolved;        // label census
+    int flag_err;
+};
+
+template <class R>
+__global__ void k_vmax_init(Scalars* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        if (sizeof(R) == 4) s->vmax_bits = __float_as_uint(Eps<float>::vmax0);
+        else s->vmax_bits = (unsigned long long)__double_as_longlong(Eps<double>::vmax0);
+    }
+}
+
+// dt = min(request or stable, cap) (src/gpk_simulation.cpp:31-40)
+template <class R>
+__global__ void k_dt(Scalars* s, R dt_req, R cap, float dx_f, float D_f, int dim3) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    R vmax;
+    if (sizeof(R) == 4) vmax = R(__uint_as_float((unsigned int)s->vmax_bits));
+    else vmax = R(__longlong_as_double((long long)s->vmax_bits));
+    const R dx = R(dx_f), D = R(D_f);
+    const R nb = dim3 ? R(6) : R(4);
+    const R dti = R(Eps<R>::c_iface) * dx / vmax;
+    const R dtd = D > R(0) ? R(Eps<R>::c_diff) * dx * dx / (nb * D) : R(Eps<R>::big);
+    const R stable = dtd < dti ? dtd : dti;
+    R dt = dt_req < R(0) ? stable : (stable < dt_req ? stable : dt_req);
+    dt = cap < dt ? cap : dt;
+    s->dt = double(dt);
+}
+
+__global__ void k_time_advance(Scalars* s) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) s->time += s->dt;
+}
+
+// ---------------------------------------------------------------------------
+// alpha integration and flip candidates (src/gpk_transitions.cpp:251-292)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_alpha(int nband, const int* __restrict__ band, const Scalars* __restrict__ s,
+                        float dx_f, const R* __restrict__ vn, R* __restrict__ alpha,
+                        const R* __restrict__ T, const int32_t* __restrict__ phi,
+                        const int32_t* __restrict__ phi_comp, uint8_t* __restrict__ cflag,
+                        R* __restrict__ anew_s, int noise, float noise_amp, uint64_t step_id) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nband) return;
+    const int i = band[a];
+    const R dt = R(s->dt);
+    const R idt = dt / R(dx_f);
+    R v = vn[a];
+    if (noise) {
+        const R Tm = T[i] < R(1) ? R(1) : T[i];
+        v += R(noise_amp) * dsqrt(Tm) * R(normal_dev(i, step_id, 0xabcddcbaULL));
+    }
+    R an = alpha[i] + v * idt;
+    an = R(1) < an ? R(1) : an;
+    const bool cand = an <= Eps<R>::flip && phi_comp[i] != phi[i];
+    if (!cand) alpha[i] = an;
+    cflag[a] = cand ? 1 : 0;
+    anew_s[a] = an;
+}
+
+// noise amplitude needs dt on the host side of the formula; computed on device
+template <class R>
+__global__ void k_noise_amp(const Scalars* s, float dx_f, float M0, float kB, float scale,
+                            float* out) {
+    if (threadIdx.x || blockIdx.x) return;
+    const float dt = float(R(s->dt));
+    const float cv = dx_f * dx_f * dx_f;
+    const float inv_dt = dt > 0.f ? 1.f / dt : 0.f;
+    const float m = cv < 1e-30f ? 1e-30f : cv;
+    *out = scale * sqrtf(2.f * M0 * kB * inv_dt / m);
+}
+
+// candidate table: cell, old/new label, new orientation, alpha_new
+template <class R>
+__global__ void k_cand_init(int m, const int* __restrict__ sel, const int* __restrict__ band,
+                            const int32_t* __restrict__ phi, const int32_t* __restrict__ phi_comp,
+                            const R* __restrict__ g_comp, const R* __restrict__ anew_s,
+                            int* __restrict__ Ccell, int* __restrict__ Cold, int* __restrict__ Cnew,
+                            R* __restrict__ Cg, R* __restrict__ Canew, int* __restrict__ state,
+                            int* __restrict__ cand_id, unsigned long long* __restrict__ Pk) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int sa = sel[k];
+    const int cell = band[sa];
+    const int32_t ol = phi[cell], nl = phi_comp[cell];
+    Ccell[k] = cell; Cold[k] = ol; Cnew[k] = nl;
+    Cg[k] = g_comp[cell]; Canew[k] = anew_s[sa];
+    state[k] = UNDEC;
+    cand_id[cell] = k;
+    if (Pk) {
+        Pk[2 * k] = ((unsigned long long)(uint32_t)ol << 32) | (uint32_t)k;
+        Pk[2 * k + 1] = ((unsigned long long)(uint32_t)nl << 32) | (uint32_t)k;
+    }
+}
+
+__global__ void k_cand_pos(int n2, const unsigned long long* __restrict__ Ps,
+                           const int* __restrict__ Cold, int* __restrict__ pos_old,
+                           int* __restrict__ pos_new) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n2) return;
+    const unsigned long long key = Ps[p];
+    const int k = int(uint32_t(key));
+    const int32_t lab = int32_t(uint32_t(key >> 32));
+    if (lab == Cold[k]) pos_old[k] = p; else pos_new[k] = p;
+}
+
+// label of `cell` as seen by transaction k (earlier accepted flips applied)
+__device__ __forceinline__ int32_t view_label(int cell, int k, const int32_t* phi,
+                                              const int* cand_id, const int* state,
+                                              const int* Cnew) {
+    const int j = cand_id[cell];
+    if (j >= 0 && j < k && ld_acq(state + j) == ACCEPT) return Cnew[j];
+    return phi[cell];
+}
+
+struct FlipCtx {
+    Dims d;
+    int dim3;
+    int m;
+    const int* Ccell; const int* Cold; const int* Cnew;
+    int* state;
+    const int* cand_id;
+    const int32_t* phi;
+    const unsigned long long* Ps;
+    const int* pos_old; const int* pos_new;
+    const int* count;          // label counts at the start of the flip pass
+    int* queue[2]; int* qflags;
+    Scalars* s;
+};
+
+__device__ __forceinline__ void decide(FlipCtx& c, int k, int v) {
+    st_rel(c.state + k, v);
+    atomicSub(&c.s->undecided, 1);
+}
+
+// all earlier transactions touching label L decided?  (label-ordered prefix)
+__device__ __forceinline__ bool label_prefix_ready(const FlipCtx& c, int pos, int32_t L) {
+    for (int q = pos - 1; q >= 0; --q) {
+        const unsigned long long key = c.Ps[q];
+        if (int32_t(uint32_t(key >> 32)) != L) break;
+        if (ld_acq(c.state + int(uint32_t(key))) == UNDEC ||
+            ld_acq(c.state + int(uint32_t(key))) == QUEUED) return false;
+    }
+    return true;
+}
+
+// count of label L as seen by transaction at label-list position pos
+__device__ __forceinline__ int label_count_at(const FlipCtx& c, int pos, int32_t L) {
+    int n = c.count[L];
+    for (int q = pos - 1; q >= 0; --q) {
+        const unsigned long long key = c.Ps[q];
+        if (int32_t(uint32_t(key >> 32)) != L) break;
+        const int j = int(uint32_t(key));
+        if (ld_acq(c.state + j) == ACCEPT) n += (c.Cnew[j] == L) ? 1 : -1;
+    }
+    return n;
+}
+
+// One round of the ordered flip transaction (src/gpk_transitions.cpp:288-313,
+// certificate src/gpk_connectivity.cpp:175-194).  A candidate is decided once
+// every earlier candidate that can change its inputs is decided:
+//   - earlier candidates in its 3^d cube sharing a label (topological numbers);
+//   - earlier candidates sharing the tested label (count, global BFS).
+// Candidates that need a BFS are queued for k_flip_bfs.
+__global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c.m) return;
+    if (ld_acq(c.state + k) != UNDEC) return;
+    const Dims& d = c.d;
+    const int cell = c.Ccell[k];
+    const int32_t ol = c.Cold[k], nl = c.Cnew[k];
+    int z0, y0, x0;
+    decode(d, cell, z0, y0, x0);
+    const int zb = c.dim3 ? -1 : 0, ze = c.dim3 ? 1 : 0;
+    uint32_t occ_o = 0, occ_n = 0;
+    int t = 0;
+    for (int dz = zb; dz <= ze; ++dz) {
+        int z = z0 + dz; if (z < 0) z += d.Nz; else if (z >= d.Nz) z -= d.Nz;
+        for (int dy = -1; dy <= 1; ++dy) {
+            int y = y0 + dy; if (y < 0) y += d.Ny; else if (y >= d.Ny) y -= d.Ny;
+            for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
+                int x = x0 + ddx; if (x < 0) x += d.Nx; else if (x >= d.Nx) x -= d.Nx;
+                const int nb = lin(d, z, y, x);
+                if (nb == cell && dz == 0 && dy == 0 && ddx == 0) continue;
+                const int j = c.cand_id[nb];
+                int32_t lab = c.phi[nb];
+                if (j >= 0 && j < k) {
+                    const int jo = c.Cold[j], jn = c.Cnew[j];
+                    const int sj = ld_acq(c.state + j);
+                    if (jo == ol || jo == nl || jn == ol || jn == nl) {
+                        if (sj == UNDEC || sj == QUEUED) return;   // not ready
+                    }
+                    if (sj == ACCEPT) lab = jn;
+                }
+                if (lab == ol) occ_o |= 1u << t;
+                if (lab == nl) occ_n |= 1u << t;
+            }
+        }
+    }
+    const int centre = c.dim3 ? 13 : 4;
+    occ_o &= ~(1u << centre);
+    occ_n &= ~(1u << centre);
+    const int oo = topo_count(occ_o, c.dim3 != 0);
+    const int no = topo_count(occ_n, c.dim3 != 0);
+    if (no == 0) { decide(c, k, REJECT); return; }
+    const bool need_old = oo != 1;      // count (oo == 0) or BFS (oo > 1)
+    const bool need_new = no > 1;
+    if (!need_old && !need_new) { decide(c, k, ACCEPT); return; }
+    if (need_old && !label_prefix_ready(c, c.pos_old[k], ol)) return;
+    if (need_new && !label_prefix_ready(c, c.pos_new[k], nl)) return;
+    if (oo == 0) {
+        const int cnt = label_count_at(c, c.pos_old[k], ol);
+        if (cnt > 1) { decide(c, k, REJECT); return; }
+        if (!need_new) { decide(c, k, ACCEPT); return; }
+    }
+    const int flags = (oo > 1 ? 1 : 0) | (need_new ? 2 : 0);
+    if (atomicCAS(c.state + k, UNDEC, QUEUED) == UNDEC) {
+        c.qflags[k] = flags;
+        const int qi = atomicAdd(&c.s->qcount[round & 1], 1);
+        c.queue[round & 1][qi] = k;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-source face-connectivity test on label L around `cell` as seen by
+// transaction k: are all face branches of L (centre excluded when `excl`)
+// in one connected component?  Same answer as the reference's early-exit
+// periodic search (src/gpk_connectivity.cpp:79-115), found by growing every
+// branch at once and stopping when all colours have met or one merged colour
+// group can grow no further.  One workgroup; LDS hash of visited cells.
+// ---------------------------------------------------------------------------
+constexpr int BFS_HCAP = 8192;      // hash slots (power of two)
+constexpr int BFS_HMAX = 6144;      // max insertions before overflow
+constexpr int BFS_FCAP = 2048;      // frontier capacity
+
+struct BfsShared {
+    unsigned long long hash[BFS_HCAP];
+    int fr[2][BFS_FCAP];
+    uint8_t frc[2][BFS_FCAP];
+    int fcount[2];
+    int inserted;
+    unsigned long long adj;
+    int present;
+    int result;       // -1 running, 0 disconnected, 1 connected
+    int overflow;
+    int nbranch;
+};
+
+__device__ __forceinline__ uint32_t hslot(int cell) {
+    return (uint32_t(cell) * 2654435761u) >> (32 - 13);
+}
+
+// returns 1 connected, 0 disconnected, -1 overflow
+__device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32_t L, bool excl,
+                                 int k) {
+    const Dims& d = c.d;
+    const int nface = c.dim3 ? 6 : 4;
+    for (int h = threadIdx.x; h < BFS_HCAP; h += blockDim.x) S.hash[h] = ~0ull;
+    if (threadIdx.x == 0) {
+        S.fcount[0] = 0; S.fcount[1] = 0; S.inserted = 0; S.adj = 0ull; S.present = 0;
+        S.result = -1; S.overflow = 0; S.nbranch = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int f[6];
+        faces(d, cell, f);
+        int nb = 0;
+        for (int q = 0; q < nface; ++q) {
+            if (view_label(f[q], k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+            const int col = nb++;
+            const unsigned long long val = ((unsigned long long)(uint32_t)f[q] << 8) | col;
+            uint32_t h = hslot(f[q]);
+            while (true) {
+                const unsigned long long cur = S.hash[h];
+                if (cur == ~0ull) {
+                    S.hash[h] = val; ++S.inserted;
+                    S.fr[0][S.fcount[0]] = f[q]; S.frc[0][S.fcount[0]] = uint8_t(col);
+                    ++S.fcount[0];
+                    break;
+                }
+                if (int(cur >> 8) == f[q]) {
+                    const int oc = int(cur & 0xff);
+                    S.adj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
+                    break;
+                }
+                h = (h + 1) & (BFS_HCAP - 1);
+            }
+        }
+        S.nbranch = nb;
+        if (nb <= 1) S.result = 1;
+    }
+    __syncthreads();
+    int cur = 0;
+    while (true) {
+        if (threadIdx.x == 0 && S.result < 0) {
+            // union colours through recorded contacts
+            const int nb = S.nbranch;
+            int comp[6];
+            for (int q = 0; q < nb; ++q) comp[q] = q;
+            bool changed = true;
+            while (changed) {
+                changed = false;
+                for (int a = 0; a < nb; ++a)
+                    for (int b = 0; b < nb; ++b)
+                        if (((S.adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
+                            const int m = comp[a] < comp[b] ? comp[a] : comp[b];
+                            if (comp[a] != m || comp[b] != m) { comp[a] = m; comp[b] = m; changed = true; }
+                        }
+            }
+            bool all = true;
+            for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
+            if (S.overflow || S.fcount[cur] > BFS_FCAP) {
+                S.result = -2;
+            } else if (all) {
+                S.result = 1;
+            } else {
+                // a merged group with no frontier is closed: disconnected
+                int present = 0;
+                for (int e = 0; e < S.fcount[cur]; ++e) present |= 1 << S.frc[cur][e];
+                for (int q = 0; q < nb && S.result < 0; ++q) {
+                    bool has = false;
+                    for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1)) has = true;
+                    if (!has) S.result = 0;
+                }
+            }
+        }
+        __syncthreads();
+        if (S.result != -1) break;
+        const int nxt = cur ^ 1;
+        if (threadIdx.x == 0) S.fcount[nxt] = 0;
+        __syncthreads();
+        const int nf = S.fcount[cur] < BFS_FCAP ? S.fcount[cur] : BFS_FCAP;
+        for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+            const int cc = S.fr[cur][e];
+            const int col = S.frc[cur][e];
+            int f[6];
+            faces(d, cc, f);
+            for (int q = 0; q < nface; ++q) {
+                const int nbc = f[q];
+                if (excl && nbc == cell) continue;
+                if (view_label(nbc, k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | col;
+                uint32_t h = hslot(nbc);
+                for (int probe = 0; probe < BFS_HCAP; ++probe) {
+                    const unsigned long long prev = atomicCAS(&S.hash[h], ~0ull, val);
+                    if (prev == ~0ull) {
+                        const int ins = atomicAdd(&S.inserted, 1);
+                        const int pos = atomicAdd(&S.fcount[nxt], 1);
+                        if (ins >= BFS_HMAX || pos >= BFS_FCAP) {
+                            S.overflow = 1;
+                        } else {
+                            S.fr[nxt][pos] = nbc; S.frc[nxt][pos] = uint8_t(col);
+                        }
+                        break;
+                    }
+                    if (int(prev >> 8) == nbc) {
+                        const int oc = int(prev & 0xff);
+                        if (oc != col)
+                            atomicOr(&S.adj, (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc)));
+                        break;
+                    }
+                    h = (h + 1) & (BFS_HCAP - 1);
+                }
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0 && S.fcount[nxt] > BFS_FCAP) S.overflow = 1;
+        cur = nxt;
+        __syncthreads();
+    }
+    const int r = S.result;
+    __syncthreads();
+    return r == -2 ? -1 : r;
+}
+
+__global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, int round, int* __restrict__ fbq) {
+    __shared__ BfsShared S;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.s->qcount[(round + 1) & 1] = 0;
+    const int n = c.s->qcount[round & 1];
+    for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
+        const int k = c.queue[round & 1][qi];
+        const int flags = c.qflags[k];
+        const int cell = c.Ccell[k];
+        int ok = 1;
+        if (flags & 1) ok = bfs_connected_lds(S, c, cell, c.Cold[k], true, k);
+        if (ok == 1 && (flags & 2)) ok = bfs_connected_lds(S, c, cell, c.Cnew[k], false, k);
+        if (threadIdx.x == 0) {
+            if (ok < 0) {
+                const int p = atomicAdd(&c.s->fbcount, 1);
+                fbq[p] = k;
+            } else {
+                decide(c, k, ok ? ACCEPT : REJECT);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Large-component fallback: the same multi-source test with the visited set
+// in device memory and frontiers of grid capacity.  A cell's mark packs the
+// transaction tag and the colour ((tag << 3) | colour), so no clearing is
+// needed between tests.  Runs as a single workgroup over the fallback queue.
+__device__ __forceinline__ int claim_mark(int* mark, int cell, int tag, int col, int* other) {
+    int old = __hip_atomic_load(mark + cell, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (true) {
+        if ((old >> 3) == tag) { *other = old & 7; return 0; }
+        const int prev = atomicCAS(mark + cell, old, (tag << 3) | col);
+        if (prev == old) return 1;
+        old = prev;
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
+               int* __restrict__ fr0, int* __restrict__ fr1, int* __restrict__ tag_ctr) {
+    __shared__ int fcount[2];
+    __shared__ unsigned long long adj;
+    __shared__ int result, nbranch, tag;
+    const Dims& d = c.d;
+    const int nface = c.dim3 ? 6 : 4;
+    const int n = c.s->fbcount;
+    for (int qi = 0; qi < n; ++qi) {
+        const int k = fbq[qi];
+        const int flags = c.qflags[k];
+        const int cell = c.Ccell[k];
+        int ok = 1;
+        for (int pass = 0; pass < 2 && ok == 1; ++pass) {
+            if (pass == 0 && !(flags & 1)) continue;
+            if (pass == 1 && !(flags & 2)) continue;
+            const int32_t L = pass == 0 ? c.Cold[k] : c.Cnew[k];
+            const bool excl = pass == 0;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                tag = ++(*tag_ctr);
+                fcount[0] = 0; fcount[1] = 0; adj = 0ull; result = -1;
+                int f[6];
+                faces(d, cell, f);
+                int nb = 0;
+                for (int q = 0; q < nface; ++q) {
+                    if (view_label(f[q], k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                    const int col = nb++;
+                    int oc = 0;
+                    if (claim_mark(mark, f[q], tag, col, &oc)) fr0[fcount[0]++] = f[q];
+                    else adj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
+                }
+                nbranch = nb;
+                if (nb <= 1) result = 1;
+            }
+            __syncthreads();
+            int cur = 0;
+            while (true) {
+                if (threadIdx.x == 0 && result < 0) {
+                    const int nb = nbranch;
+                    int comp[6];
+                    for (int q = 0; q < nb; ++q) comp[q] = q;
+                    bool changed = true;
+                    while (changed) {
+                        changed = false;
+                        for (int a = 0; a < nb; ++a)
+                            for (int b = 0; b < nb; ++b)
+                                if (((adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
+                                    const int m = comp[a] < comp[b] ? comp[a] : comp[b];
+                                    comp[a] = m; comp[b] = m; changed = true;
+                                }
+                    }
+                    bool all = true;
+                    for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
+                    if (all) result = 1;
+                    else {
+                        int pres = 0;
+                        const int* frc = cur == 0 ? fr0 : fr1;
+                        for (int e = 0; e < fcount[cur]; ++e)
+                            pres |= 1 << (__hip_atomic_load(mark + frc[e], __ATOMIC_RELAXED,
+                                                            __HIP_MEMORY_SCOPE_AGENT) & 7);
+                        for (int q = 0; q < nb && result < 0; ++q) {
+                            bool has = false;
+                            for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((pres >> p) & 1)) has = true;
+                            if (!has) result = 0;
+                        }
+                    }
+                    fcount[cur ^ 1] = 0;
+                }
+                __syncthreads();
+                if (result != -1) break;
+                const int nxt = cur ^ 1;
+                const int* frc = cur == 0 ? fr0 : fr1;
+                int* frn = nxt == 0 ? fr0 : fr1;
+                const int nf = fcount[cur];
+                const int tg = tag;
+                for (int e = threadIdx.x; e < nf; e += blockDim.x) {
+                    const int cc = frc[e];
+                    const int col = __hip_atomic_load(mark + cc, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT) & 7;
+                    int f[6];
+                    faces(d, cc, f);
+                    for (int q = 0; q < nface; ++q) {
+                        const int nbc = f[q];
+                        if (excl && nbc == cell) continue;
+                        if (view_label(nbc, k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                        int oc = 0;
+                        if (claim_mark(mark, nbc, tg, col, &oc)) {
+                            frn[atomicAdd(&fcount[nxt], 1)] = nbc;
+                        } else if (oc != col) {
+                            atomicOr(&adj, (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc)));
+                        }
+                    }
+                }
+                __threadfence();
+                __syncthreads();
+                cur = nxt;
+            }
+            ok = result;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) decide(c, k, ok ? ACCEPT : REJECT);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) c.s->fbcount = 0;
+}
+
+// apply the decided transactions (src/gpk_transitions.cpp:300-312)
+template <class R>
+__global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __restrict__ Cold,
+                             const int* __restrict__ Cnew, const R* __restrict__ Cg,
+                             const R* __restrict__ Canew, const int* __restrict__ state,
+                             int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
+                             int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
+                             int all_accept) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const int cell = Ccell[k];
+    cand_id[cell] = -1;
+    if (all_accept || state[k] == ACCEPT) {
+        phi[cell] = Cnew[k];
+        g[cell] = Cg[k];
+        alpha[cell] = R(1);
+        atomicSub(count + Cold[k], 1);
+        atomicAdd(count + Cnew[k], 1);
+        atomicAdd(&s->accepted, 1);
+    } else {
+        const R an = Canew[k];
+        alpha[cell] = an < Eps<R>::flip ? Eps<R>::flip : an;
+        atomicAdd(&s->deferred, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Balanced Gaussian remap (src/gpk_transitions.cpp:38-178)
+// ---------------------------------------------------------------------------
+template <class R, int DIM>
+__global__ void __launch_bounds__(256)
+k_remap_score(Dims d, int r, int nband, const int* __restrict__ band,
+              const int32_t* __restrict__ phi, const double* __restrict__ wtab,
+              int32_t* __restrict__ phi_comp, uint8_t* __restrict__ pflag,
+              R* __restrict__ gain_s) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nband) return;
+    const int i = band[a];
+    const int32_t p = phi[i];
+    int f[6];
+    faces(d, i, f);
+    const int nface = DIM == 3 ? 6 : 4;
+    int32_t cand[6];
+    int nc = 0;
+    for (int k = 0; k < nface; ++k) {
+        const int32_t v = phi[f[k]];
+        if (v == p) continue;
+        bool known = false;
+        for (int q = 0; q < nc; ++q) known |= cand[q] == v;
+        if (!known) cand[nc++] = v;
+    }
+    int z0, y0, x0;
+    decode(d, i, z0, y0, x0);
+    double resident = 0.0, sup[6] = {0, 0, 0, 0, 0, 0};
+    const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+    for (int dz = dzb; dz <= dze; ++dz) {
+        const int z = wrapi(z0 + dz, d.Nz);
+        for (int dy = -r; dy <= r; ++dy) {
+            const int y = wrapi(y0 + dy, d.Ny);
+            for (int ddx = -r; ddx <= r; ++ddx) {
+                const int x = wrapi(x0 + ddx, d.Nx);
+                const int32_t v = phi[lin(d, z, y, x)];
+                const double w = wtab[ddx * ddx + dy * dy + dz * dz];
+                if (v == p) resident += w;
+                for (int q = 0; q < nc; ++q) if (v == cand[q]) sup[q] += w;
+            }
+        }
+    }
+    double best = -1.0;
+    int32_t bl = p;
+    for (int q = 0; q < nc; ++q) if (sup[q] > best) { best = sup[q]; bl = cand[q]; }
+    phi_comp[i] = bl;
+    const R es = R(resident), eb = R(best);
+    const R gain = eb - es;
+    pflag[a] = (p != bl && gain > Eps<R>::gain) ? 1 : 0;
+    gain_s[a] = gain;
+}
+
+__global__ void k_copy_i32(int n, const int32_t* __restrict__ src, int32_t* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+// proposal keys: first pass by gain (descending) then cell, second by
+// (low, high, direction)
+template <class R>
+__global__ void k_prop_keys(int np, const int* __restrict__ sel, const int* __restrict__ band,
+                            const R* __restrict__ gain_s, const int32_t* __restrict__ phi,
+                            const int32_t* __restrict__ phi_comp, int* __restrict__ pcell,
+                            unsigned long long* __restrict__ kgain, int* __restrict__ idx) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= np) return;
+    const int sa = sel[k];
+    const int cell = band[sa];
+    pcell[k] = cell;
+    idx[k] = k;
+    const R gn = gain_s[sa];
+    unsigned long long gb;
+    if (sizeof(R) == 4) gb = (unsigned long long)(~__float_as_uint(float(gn)));
+    else gb = ~(unsigned long long)__double_as_longlong(double(gn));
+    kgain[k] = gb;
+}
+
+__global__ void k_prop_group_keys(int np, const int* __restrict__ order,
+                                  const int* __restrict__ pcell, const int32_t* __restrict__ phi,
+                                  const int32_t* __restrict__ phi_comp,
+                                  unsigned long long* __restrict__ kgrp) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const int cell = pcell[order[s]];
+    const int32_t p = phi[cell], q = phi_comp[cell];
+    const uint32_t lo = uint32_t(p < q ? p : q), hi = uint32_t(p < q ? q : p);
+    const uint32_t dir = p == int32_t(lo) ? 0u : 1u;
+    kgrp[s] = ((unsigned long long)lo << 27 | hi) << 1 | dir;
+}
+
+// segments of equal (low, high, dir): start index per position
+__global__ void k_seg_bounds(int np, const unsigned long long* __restrict__ kg,
+                             int* __restrict__ seg_start, int* __restrict__ seg_end) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    const bool first = s == 0 || kg[s] != kg[s - 1];
+    const bool last = s == np - 1 || kg[s] != kg[s + 1];
+    if (first) {
+        int e = s;
+        while (e + 1 < np && kg[e + 1] == kg[s]) ++e;
+        for (int t = s; t <= e; ++t) { seg_start[t] = s; seg_end[t] = e; }
+    }
+    (void)last;
+}
+
+// forward entry at rank r pairs with the reverse entry of the same rank
+__global__ void k_pair_flags(int np, const unsigned long long* __restrict__ kg,
+                             const int* __restrict__ seg_start, const int* __restrict__ seg_end,
+                             int* __restrict__ pflag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np) return;
+    int f = 0;
+    if ((kg[s] & 1ull) == 0ull) {
+        const int e = seg_end[s];
+        const int rnk = s - seg_start[s];
+        if (e + 1 < np && (kg[e + 1] >> 1) == (kg[s] >> 1)) {
+            const int rs = e + 1;
+            const int re = seg_end[rs];
+            if (rnk <= re - rs) f = 1;
+        }
+    }
+    pflag[s] = f;
+}
+
+template <class R>
+__global__ void k_pair_emit(int np, const unsigned long long* __restrict__ kg,
+                            const int* __restrict__ seg_start, const int* __restrict__ seg_end,
+                            const int* __restrict__ pflag, const int* __restrict__ pidx,
+                            const int* __restrict__ order, const int* __restrict__ pcell,
+                            int* __restrict__ Pa, int* __restrict__ Pb, int* __restrict__ Plo,
+                            int* __restrict__ Phi, int* __restrict__ pstate,
+                            int* __restrict__ pid) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= np || !pflag[s]) return;
+    const int rnk = s - seg_start[s];
+    const int rs = seg_end[s] + 1;
+    const int q = pidx[s];
+    const int a = pcell[order[s]], b = pcell[order[rs + rnk]];
+    const unsigned long long key = kg[s] >> 1;
+    Pa[q] = a; Pb[q] = b;
+    Plo[q] = int(uint32_t(key >> 27));
+    Phi[q] = int(uint32_t(key & ((1ull << 27) - 1)));
+    pstate[q] = UNDEC;
+    pid[a] = q; pid[b] = q;
+}
+
+struct PairCtx {
+    Dims d;
+    int dim3;
+    int np;
+    const int* Pa; const int* Pb; const int* Plo; const int* Phi;
+    int* pstate;
+    const int* pid;
+    const int32_t* phi;
+    const void* g;          // R*
+    void* Pg;               // R*, 2 per pair (ga, gb)
+    const int* count;
+    Scalars* s;
+};
+
+__device__ __forceinline__ float ld_agent(const float* p) {
+    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const unsigned int*>(p),
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ double ld_agent(const double* p) {
+    return __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_agent(float* p, float v) {
+    __hip_atomic_store(reinterpret_cast<unsigned int*>(p), __float_as_uint(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_agent(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p),
+                       (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class R>
+__device__ __forceinline__ void pair_view(const PairCtx& c, int cell, int q, int32_t& lab, R& gv) {
+    const int p = c.pid[cell];
+    const R* g = static_cast<const R*>(c.g);
+    if (p >= 0 && p < q && __hip_atomic_load(c.pstate + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == ACCEPT) {
+        const R* Pg = static_cast<const R*>(c.Pg);
+        if (c.Pa[p] == cell) { lab = c.Phi[p]; gv = ld_agent(Pg + 2 * p); }
+        else { lab = c.Plo[p]; gv = ld_agent(Pg + 2 * p + 1); }
+        return;
+    }
+    lab = c.phi[cell];
+    gv = g[cell];
+}
+
+__device__ __forceinline__ int32_t pair_label(const PairCtx& c, int cell, int q) {
+    const int p = c.pid[cell];
+    if (p >= 0 && p < q && ld_acq(c.pstate + p) == ACCEPT)
+        return c.Pa[p] == cell ? c.Phi[p] : c.Plo[p];
+    return c.phi[cell];
+}
+
+// bounded certificate for relabelling `cell` old->nl, with one override
+// (cell ov carries label ovl), src/gpk_connectivity.cpp:10-195 (local_only)
+__device__ bool bounded_move(const PairCtx& c, int q, int cell, int32_t ol, int32_t nl,
+                             int old_count, int ov, int32_t ovl) {
+    const Dims& d = c.d;
+    const bool d3 = c.dim3 != 0;
+    int z0, y0, x0;
+    decode(d, cell, z0, y0, x0);
+    Bits128 bo{0, 0}, bn{0, 0};
+    const int nz = d3 ? 5 : 1;
+    for (int lz = 0; lz < nz; ++lz) {
+        const int z = wrapi(z0 + (d3 ? lz - 2 : 0), d.Nz);
+        for (int ly = 0; ly < 5; ++ly) {
+            const int y = wrapi(y0 + ly - 2, d.Ny);
+            for (int lx = 0; lx < 5; ++lx) {
+                const int x = wrapi(x0 + lx - 2, d.Nx);
+                const int cc = lin(d, z, y, x);
+                const int32_t lab = cc == ov ? ovl : (cc == cell ? ol : pair_label(c, cc, q));
+                const int b = lz * 25 + ly * 5 + lx;
+                if (lab == ol) b_set(bo, b);
+                if (lab == nl) b_set(bn, b);
+            }
+        }
+    }
+    const int lc = (d3 ? 2 : 0) * 25 + 12;
+    // 3^d cube occupancy from the box
+    uint32_t co = 0, cn = 0;
+    {
+        int t = 0;
+        const int zb = d3 ? -1 : 0, ze = d3 ? 1 : 0;
+        for (int dz = zb; dz <= ze; ++dz)
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
+                    const int b = lc + dz * 25 + dy * 5 + ddx;
+                    if (b_test(bo, b)) co |= 1u << t;
+                    if (b_test(bn, b)) cn |= 1u << t;
+                }
+    }
+    const int centre = d3 ? 13 : 4;
+    co &= ~(1u << centre);
+    cn &= ~(1u << centre);
+    const BoxMasks M = box_masks(d3);
+    const int bl[6] = {lc - 1, lc + 1, lc - 5, lc + 5, lc - 25, lc + 25};
+    const int nface = d3 ? 6 : 4;
+    auto connected = [&](Bits128 occ) -> bool {
+        int br[6], nb = 0;
+        for (int k = 0; k < nface; ++k) if (b_test(occ, bl[k])) br[nb++] = bl[k];
+        if (nb <= 1) return true;
+        Bits128 comp{0, 0};
+        b_set(comp, br[0]);
+        while (true) {
+            const Bits128 g = b_and(grow5(comp, M, d3), occ);
+            if (b_eq(g, comp)) break;
+            comp = g;
+        }
+        for (int k = 1; k < nb; ++k) if (!b_test(comp, br[k])) return false;
+        return true;
+    };
+    const int oo = topo_count(co, d3);
+    if (oo == 0 && !(old_count <= 1)) return false;
+    if (oo > 1) {
+        Bits128 occ = bo;
+        // centre excluded for the old label
+        Bits128 cm{0, 0};
+        b_set(cm, lc);
+        occ = b_andn(occ, cm);
+        if (!connected(occ)) return false;
+    }
+    const int no = topo_count(cn, d3);
+    if (no == 0) return false;
+    if (no > 1 && !connected(bn)) return false;
+    return true;
+}
+
+template <class R>
+__global__ void __launch_bounds__(256) k_pair_round(PairCtx c) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= c.np) return;
+    if (ld_acq(c.pstate + q) != UNDEC) return;
+    const Dims& d = c.d;
+    const bool d3 = c.dim3 != 0;
+    const int a = c.Pa[q], b = c.Pb[q];
+    const int32_t lo = c.Plo[q], hi = c.Phi[q];
+    // readiness: earlier pairs sharing a label inside the 5^d boxes of a and b
+    for (int side = 0; side < 2; ++side) {
+        int z0, y0, x0;
+        decode(d, side == 0 ? a : b, z0, y0, x0);
+        const int nz = d3 ? 5 : 1;
+        for (int lz = 0; lz < nz; ++lz) {
+            const int z = wrapi(z0 + (d3 ? lz - 2 : 0), d.Nz);
+            for (int ly = 0; ly < 5; ++ly) {
+                const int y = wrapi(y0 + ly - 2, d.Ny);
+                for (int lx = 0; lx < 5; ++lx) {
+                    const int x = wrapi(x0 + lx - 2, d.Nx);
+                    const int p = c.pid[lin(d, z, y, x)];
+                    if (p >= 0 && p < q) {
+                        const int32_t pl = c.Plo[p], ph = c.Phi[p];
+                        if ((pl == lo || pl == hi || ph == lo || ph == hi) &&
+                            ld_acq(c.pstate + p) == UNDEC) return;
+                    }
+                }
+            }
+        }
+    }
+    R* Pg = static_cast<R*>(c.Pg);
+    const int nface = d3 ? 6 : 4;
+    int32_t la, lb;
+    R gva, gvb;
+    pair_view<R>(c, a, q, la, gva);
+    pair_view<R>(c, b, q, lb, gvb);
+    int verdict = REJECT;
+    if (la == lo && lb == hi) {
+        int fa[6], fb[6];
+        faces(d, a, fa);
+        faces(d, b, fb);
+        // target orientations (first face neighbour carrying the target label)
+        R ga = gva, gb = gvb;
+        for (int k = 0; k < nface; ++k) {
+            int32_t l; R gv; pair_view<R>(c, fa[k], q, l, gv);
+            if (l == hi) { ga = gv; break; }
+        }
+        for (int k = 0; k < nface; ++k) {
+            int32_t l; R gv; pair_view<R>(c, fb[k], q, l, gv);
+            if (l == lo) { gb = gv; break; }
+        }
+        auto mism = [&](const int* f, int32_t prop, int pc, int32_t pp) {
+            int m = 0;
+            for (int k = 0; k < nface; ++k) {
+                const int32_t l = f[k] == pc ? pp : pair_label(c, f[k], q);
+                m += l != prop;
+            }
+            return m;
+        };
+        const int before = mism(fa, lo, b, hi) + mism(fb, hi, a, lo);
+        const int after = mism(fa, hi, b, lo) + mism(fb, lo, a, hi);
+        if (after < before &&
+            bounded_move(c, q, a, lo, hi, c.count[lo], -1, 0) &&
+            bounded_move(c, q, b, hi, lo, c.count[hi] + 1, a, hi)) {
+            verdict = ACCEPT;
+            st_agent(Pg + 2 * q, ga);
+            st_agent(Pg + 2 * q + 1, gb);
+        }
+    }
+    __hip_atomic_store(c.pstate + q, verdict, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    atomicSub(&c.s->undecided, 1);
+}
+
+template <class R>
+__global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __restrict__ Pb,
+                             const int* __restrict__ Plo, const int* __restrict__ Phi,
+                             const int* __restrict__ pstate, const R* __restrict__ Pg,
+                             int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
+                             int* __restrict__ pid, Scalars* s) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= np) return;
+    const int a = Pa[q], b = Pb[q];
+    pid[a] = -1; pid[b] = -1;
+    if (pstate[q] != ACCEPT) return;
+    phi[a] = Phi[q]; phi[b] = Plo[q];
+    g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
+    alpha[a] = R(1); alpha[b] = R(1);
+    atomicAdd(&s->exchanges, 2);
+}
+
+// ---------------------------------------------------------------------------
+// Unresolved-grain extinction (src/gpk_transitions.cpp:180-249)
+// ---------------------------------------------------------------------------
+__global__ void k_label_census(int nlab, const int* __restrict__ count, int min_cells,
+                               uint8_t* __restrict__ unres, Scalars* s) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nlab) return;
+    const int n = count[l];
+    const bool u = n > 0 && n < min_cells;
+    unres[l] = u ? 1 : 0;
+    if (n > 0) atomicAdd(&s->present, 1);
+    if (u) atomicAdd(&s->unresolved, 1);
+}
+
+__global__ void k_unres_flags(int N, const int32_t* __restrict__ phi,
+                              const uint8_t* __restrict__ unres, uint8_t* __restrict__ flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        flag[i] = unres[phi[i]];
+}
+
+// layer-by-layer erosion from an immutable snapshot per layer (single workgroup)
+template <class R>
+__global__ void __launch_bounds__(1024)
+k_erode(Dims d, int dim3, int n, const int* __restrict__ cells, const uint8_t* __restrict__ unres,
+        int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
+        int* __restrict__ count, int* __restrict__ tgt, R* __restrict__ tgg, Scalars* s) {
+    __shared__ int changed;
+    const int nface = dim3 ? 6 : 4;
+    while (true) {
+        if (threadIdx.x == 0) changed = 0;
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            const int i = cells[e];
+            tgt[e] = -1;
+            if (!unres[phi[i]]) continue;
+            int z, y, x;
+            decode(d, i, z, y, x);
+            int f[6];
+            faces(d, i, f);
+            int best = -1, bs = -1;
+            for (int k = 0; k < nface; ++k) {
+                const int32_t q = phi[f[k]];
+                if (unres[q]) continue;
+                int sup = 0;
+                for (int dz = (dim3 ? -1 : 0); dz <= (dim3 ? 1 : 0); ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int ddx = -1; ddx <= 1; ++ddx) {
+                            const int zz = (z + dz + d.Nz) % d.Nz, yy = (y + dy + d.Ny) % d.Ny,
+                                      xx = (x + ddx + d.Nx) % d.Nx;
+                            sup += phi[lin(d, zz, yy, xx)] == q;
+                        }
+                if (sup > bs) { best = k; bs = sup; }
+            }
+            if (best >= 0) {
+                tgt[e] = phi[f[best]];
+                tgg[e] = g[f[best]];
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < n; e += blockDim.x) {
+            if (tgt[e] < 0) continue;
+            const int i = cells[e];
+            atomicSub(count + phi[i], 1);
+            atomicAdd(count + tgt[e], 1);
+            phi[i] = tgt[e];
+            g[i] = tgg[e];
+            alpha[i] = R(1);
+            atomicAdd(&changed, 1);
+        }
+        __syncthreads();
+        const int ch = changed;
+        if (threadIdx.x == 0) s->eroded += ch;
+        __syncthreads();
+        if (ch == 0) break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Solute transport (src/gpk_transport.cpp:11-135)
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_diffusion(Dims d, int dim3, const R* __restrict__ c, const uint8_t* __restrict__ mask,
+                            const R* __restrict__ alpha, const Scalars* __restrict__ s, float dx_f,
+                            float D_f, R* __restrict__ cout) {
+    const R dt = R(s->dt), dx = R(dx_f), D = R(D_f);
+    const R dx2 = dx * dx;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        int f[6];
+        faces(d, i, f);
+        R lap = c[f[1]] + c[f[0]] + c[f[3]] + c[f[2]];
+        if (dim3) lap += c[f[5]] + c[f[4]] - R(6) * c[i];
+        else lap -= R(4) * c[i];
+        const R De = mask[i] ? D * alpha[i] : D;
+        cout[i] = c[i] + dt * De * lap / dx2;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R solid_frac(int32_t p, R a) { return p != 0 ? a : R(1) - a; }
+
+template <class R>
+__global__ void k_solid_fraction(int N, const int32_t* __restrict__ phi, const R* __restrict__ alpha,
+                                 R* __restrict__ s) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        s[i] = solid_frac<R>(phi[i], alpha[i]);
+}
+
+// faces in the transport order x+, x-, y+, y-, z+, z-
+__device__ __forceinline__ int part_faces(const Dims& d, int dim3, int i, int nb[6]) {
+    int z, y, x;
+    decode(d, i, z, y, x);
+    int n = 0;
+    nb[n++] = lin(d, z, y, x == d.Nx - 1 ? 0 : x + 1);
+    nb[n++] = lin(d, z, y, x == 0 ? d.Nx - 1 : x - 1);
+    nb[n++] = lin(d, z, y == d.Ny - 1 ? 0 : y + 1, x);
+    nb[n++] = lin(d, z, y == 0 ? d.Ny - 1 : y - 1, x);
+    if (dim3) {
+        nb[n++] = lin(d, z == d.Nz - 1 ? 0 : z + 1, y, x);
+        nb[n++] = lin(d, z == 0 ? d.Nz - 1 : z - 1, y, x);
+    }
+    return n;
+}
+
+// per-source rejected solute q and liquid weight sum (src/gpk_transport.cpp:64-89)
+template <class R>
+__global__ void k_reject_source(Dims d, int dim3, float k_f, const int32_t* __restrict__ phi,
+                                const R* __restrict__ alpha, const R* __restrict__ s_old,
+                                const R* __restrict__ c, R* __restrict__ q_out,
+                                R* __restrict__ wsum_out) {
+    const R k = R(k_f);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        R qv = R(0), ws = R(0);
+        const R sn = solid_frac<R>(phi[i], alpha[i]);
+        const R ds = sn - s_old[i];
+        if (ds > R(0)) {
+            R s0 = s_old[i];
+            s0 = s0 < R(0) ? R(0) : (R(1) < s0 ? R(1) : s0);
+            R den = R(1) - s0 * (R(1) - k);
+            den = den < Eps<R>::cl_floor ? Eps<R>::cl_floor : den;
+            const R cL = c[i] / den;
+            const R dsc = R(1) < ds ? R(1) : ds;
+            R q = (R(1) - k) * cL * dsc;
+            q = c[i] < q ? c[i] : q;
+            if (q > R(0)) {
+                int nb[6];
+                const int nn = part_faces(d, dim3, i, nb);
+                for (int a = 0; a < nn; ++a) {
+                    const int j = nb[a];
+                    R w = R(0);
+                    if (phi[j] == 0) {
+                        const R l = R(1) - solid_frac<R>(phi[j], alpha[j]);
+                        w = l < R(0) ? R(0) : l;
+                    }
+                    ws += w;
+                }
+                if (ws > R(0)) qv = q;
+            }
+        }
+        q_out[i] = qv;
+        wsum_out[i] = ws;
+    }
+}
+
+// ordered gather: delta[j] accumulated over sources in ascending source index,
+// each source in its own neighbour order — the serial scatter's sequence
+template <class R>
+__global__ void k_reject_gather(Dims d, int dim3, const int32_t* __restrict__ phi,
+                                const R* __restrict__ alpha, const R* __restrict__ q,
+                                const R* __restrict__ wsum, R* __restrict__ c) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.N; j += gridDim.x * blockDim.x) {
+        int nbj[6];
+        const int nn = part_faces(d, dim3, j, nbj);
+        int src[7];
+        int ns = 0;
+        src[ns++] = j;
+        for (int a = 0; a < nn; ++a) {
+            bool dup = false;
+            for (int b = 0; b < ns; ++b) dup |= src[b] == nbj[a];
+            if (!dup) src[ns++] = nbj[a];
+        }
+        for (int x = 1; x < ns; ++x)
+            for (int y = x; y > 0 && src[y - 1] > src[y]; --y) {
+                const int t = src[y]; src[y] = src[y - 1]; src[y - 1] = t;
+            }
+        R wj = R(0);
+        if (phi[j] == 0) {
+            const R l = R(1) - solid_frac<R>(phi[j], alpha[j]);
+            wj = l < R(0) ? R(0) : l;
+        }
+        R delta = R(0);
+        for (int e = 0; e < ns; ++e) {
+            const int i = src[e];
+            const R qi = q[i];
+            if (qi <= R(0)) continue;
+            if (i == j) delta -= qi;
+            if (wj > R(0)) {
+                int nbi[6];
+                const int ni = part_faces(d, dim3, i, nbi);
+                for (int a = 0; a < ni; ++a)
+                    if (nbi[a] == j) delta += qi * wj / wsum[i];
+            }
+        }
+        c[j] += delta;
+    }
+}
+
+template <class R>
+__global__ void k_liquid_comp(int N, float k_f, const int32_t* __restrict__ phi,
+                              const R* __restrict__ alpha, const R* __restrict__ c,
+                              R* __restrict__ s, R* __restrict__ cL) {
+    const R k = R(k_f);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const R sv = solid_frac<R>(phi[i], alpha[i]);
+        s[i] = sv;
+        R den = R(1) - sv * (R(1) - k);
+        den = den < Eps<R>::cl_floor ? Eps<R>::cl_floor : den;
+        cL[i] = c[i] / den;
+    }
+}
+
+template <class R>
+__global__ void k_diffusion_part(Dims d, int dim3, const R* __restrict__ s, const R* __restrict__ cL,
+                                 const R* __restrict__ c, const Scalars* __restrict__ sc,
+                                 float dx_f, float D_f, R* __restrict__ cout) {
+    const R dt = R(sc->dt), dx = R(dx_f), D = R(D_f);
+    const R coef = dt * D / (dx * dx);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        int nb[6];
+        const int nn = part_faces(d, dim3, i, nb);
+        const R li = R(1) - s[i];
+        R flux = R(0);
+        for (int a = 0; a < nn; ++a) {
+            const int j = nb[a];
+            const R lj = R(1) - s[j];
+            const R lf = lj < li ? lj : li;
+            if (lf > R(0)) flux += lf * (cL[j] - cL[i]);
+        }
+        cout[i] = c[i] + coef * flux;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Label table maintenance and diagnostics
+// ---------------------------------------------------------------------------
+__global__ void k_label_hist(int N, const int32_t* __restrict__ phi, int* __restrict__ count) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        atomicAdd(count + phi[i], 1);
+}
+
+__global__ void k_fill_i32(int n, int32_t* p, int32_t v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void k_fill_u8(int n, uint8_t* p, uint8_t v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+template <class R>
+__global__ void k_fill_r(int n, R* p, R v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ void k_minmax_i32(int N, const int32_t* __restrict__ p, int* __restrict__ out) {
+    int mn = 0x7fffffff, mx = -0x7fffffff - 1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const int v = p[i];
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+    }
+    atomicMin(out, mn);
+    atomicMax(out + 1, mx);
+}
+
+// band-compact field -> full grid (zeros outside the band)
+template <class R>
+__global__ void k_scatter_band(int nband, const int* __restrict__ band, const R* __restrict__ src,
+                               R* __restrict__ dst) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a < nband) dst[band[a]] = src[a];
+}
+
+// FP64 sum in the canonical chunk order (1024 cells, then 1024 chunks)
+template <class R>
+__global__ void k_sum_chunks(int n, const R* __restrict__ v, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nch = (n + 1023) / 1024;
+    if (b >= nch) return;
+    double s = 0.0;
+    const int e = (b + 1) * 1024 < n ? (b + 1) * 1024 : n;
+    for (int i = b * 1024; i < e; ++i) s += double(v[i]);
+    out[b] = s;
+}
+__global__ void k_sum_final(int nch, const double* __restrict__ p1, double* __restrict__ out) {
+    if (threadIdx.x || blockIdx.x) return;
+    double tot = 0.0;
+    for (int b = 0; b < nch; b += 1024) {
+        double s = 0.0;
+        const int e = b + 1024 < nch ? b + 1024 : nch;
+        for (int i = b; i < e; ++i) s += p1[i];
+        tot += s;
+    }
+    *out = tot;
+}
+
+// phase volume terms per cell (src/gpk_simulation.cpp:78-86), chunk-summed
+template <class R>
+__global__ void k_phase_terms(int N, int32_t label, const int32_t* __restrict__ phi,
+                              const int32_t* __restrict__ phi_comp, const uint8_t* __restrict__ mask,
+                              const R* __restrict__ alpha, double* __restrict__ out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nch = (N + 1023) / 1024;
+    if (b >= nch) return;
+    double s = 0.0;
+    const int e = (b + 1) * 1024 < N ? (b + 1) * 1024 : N;
+    for (int i = b * 1024; i < e; ++i) {
+        if (phi[i] == label) s += double(alpha[i]);
+        if (mask[i] && phi_comp[i] == label && phi_comp[i] != phi[i]) s += 1.0 - double(alpha[i]);
+    }
+    out[b] = s;
+}
+
+__global__ void k_count_u8(int N, const uint8_t* __restrict__ m, int* __restrict__ out) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) c += m[i] != 0;
+    atomicAdd(out, c);
+}
+__global__ void k_count_pos(int n, const int* __restrict__ cnt, int* __restrict__ out) {
+    int c = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) c += cnt[i] > 0;
+    atomicAdd(out, c);
+}
+
+// ---------------------------------------------------------------------------
+// Voronoi nearest site (demos/gpk.cpp:72-89): exact integer periodic squared
+// distance, lowest site index on ties.
+// ---------------------------------------------------------------------------
+__global__ void k_voronoi(Dims d, int ns, const int* __restrict__ sz, const int* __restrict__ sy,
+                          const int* __restrict__ sx, int32_t* __restrict__ phi) {
+    extern __shared__ int sh[];
+    int* lz = sh;
+    int* ly = sh + 1024;
+    int* lx = sh + 2048;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int z = 0, y = 0, x = 0;
+    if (i < d.N) decode(d, i, z, y, x);
+    long long bd = 0x7fffffffffffffffll;
+    int best = 0;
+    for (int base = 0; base < ns; base += 1024) {
+        const int cnt = ns - base < 1024 ? ns - base : 1024;
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+            lz[t] = sz[base + t]; ly[t] = sy[base + t]; lx[t] = sx[base + t];
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; ++t) {
+            long long a = z - lz[t]; a = a < 0 ? -a : a;
+            long long b = y - ly[t]; b = b < 0 ? -b : b;
+            long long e = x - lx[t]; e = e < 0 ? -e : e;
+            a = a < d.Nz - a ? a : d.Nz - a;
+            b = b < d.Ny - b ? b : d.Ny - b;
+            e = e < d.Nx - e ? e : d.Nx - e;
+            const long long dd = a * a + b * b + e * e;
+            if (dd < bd) { bd = dd; best = base + t; }
+        }
+    }
+    if (i < d.N) phi[i] = best + 1;
+}
+
+} // namespace gpkd

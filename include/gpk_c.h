@@ -1,0 +1,235 @@
+/*
+ * gpk_c.h — C ABI of the MI355X-native GPK per-step kinetics library
+ * (libgpk_hip.so).
+ *
+ * Plain pointers and sizes only: no C++ or torch types cross this boundary.
+ * Every entry point returns an int status (GPK_OK == 0, negative GPK_E_* on
+ * failure) and never lets an exception escape; gpk_last_error() returns a
+ * thread-local description of the last failure on the calling thread.
+ *
+ * Each function names the reference interface it replaces
+ * (paths relative to the GPK source tree).
+ *
+ * Threading: a handle owns one HIP stream and is not thread-safe; distinct
+ * handles may be driven from distinct host threads.  Work is asynchronous on
+ * the handle's stream until a download, a diagnostic or gpk_sync().
+ */
+#ifndef GPK_C_H
+#define GPK_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum gpk_status {
+    GPK_OK                   =  0,
+    GPK_E_INVALID            = -1,  /* bad argument (null, size mismatch, range) */
+    GPK_E_HIP                = -2,  /* HIP runtime failure (message has details) */
+    GPK_E_UNSUPPORTED_MODEL  = -3,  /* model kind/flag not available on device */
+    GPK_E_LABEL_RANGE        = -4,  /* labels outside [0, GPK_MAX_LABEL] */
+    GPK_E_OOM                = -5,  /* device allocation failed */
+    GPK_E_NO_DEVICE          = -6,  /* no HIP device visible */
+    GPK_E_STALLED            = -7   /* Voronoi connectivity normalisation stalled */
+};
+
+/* Labels are dense non-negative integers; per-label tables are sized to the
+ * largest label present (plus headroom), capped at this value. */
+#define GPK_MAX_LABEL (1 << 26)
+
+/* ---- arithmetic modes ------------------------------------------------- */
+enum gpk_precision {
+    /* FP32 fields, geometry as the reference's direct masked stencil with the
+     * reference's operation order: bit-exact with the reference CPU step. */
+    GPK_PREC_STRICT32 = 0,
+    /* FP32 fields, geometry through separable row sums with FMA
+     * (fewer operations; compared to FP64 by error statistics). */
+    GPK_PREC_FP32     = 1,
+    /* FP64 fields and weights, the same separable geometry order as FP32. */
+    GPK_PREC_FP64     = 2
+};
+
+/* ---- state fields ----------------------------------------------------- */
+/* dtype: I32 = int32, U8 = uint8, R = float (STRICT32/FP32) or double (FP64) */
+enum gpk_field {
+    GPK_FIELD_PHI      = 0,   /* I32  grain / phase label            */
+    GPK_FIELD_ALPHA    = 1,   /* R    sub-cell coordinate            */
+    GPK_FIELD_C        = 2,   /* R    composition                    */
+    GPK_FIELD_T        = 3,   /* R    temperature                    */
+    GPK_FIELD_G        = 4,   /* R    orientation                    */
+    GPK_FIELD_RHO      = 5,   /* R    dislocation density (carried)  */
+    GPK_FIELD_NX       = 6,   /* R    outward normal (band cells)    */
+    GPK_FIELD_NY       = 7,
+    GPK_FIELD_NZ       = 8,
+    GPK_FIELD_KAPPA    = 9,   /* R    curvature (band cells)         */
+    GPK_FIELD_VN       = 10,  /* R    projected normal velocity      */
+    GPK_FIELD_PHI_COMP = 11,  /* I32  competitor label               */
+    GPK_FIELD_G_COMP   = 12,  /* R    competitor orientation         */
+    GPK_FIELD_MASK     = 13,  /* U8   interface mask                 */
+    GPK_FIELD_COUNT    = 14
+};
+
+/* ---- constitutive models (POD replacement of the ThermoModel plugin) --- */
+enum gpk_model_kind {
+    GPK_MODEL_ISOTROPIC      = 0,  /* IsotropicGrowthThermo   include/gpk_thermo.hpp:18-27  */
+    GPK_MODEL_ANISO_GROWTH   = 1,  /* AnisotropicGrowthThermo include/gpk_thermo.hpp:33-45  */
+    GPK_MODEL_SOLIDIFICATION = 2,  /* SolidificationThermo    include/gpk_thermo.hpp:52-85  */
+    GPK_MODEL_TENSOR_ANISO   = 3,  /* TensorAnisotropyThermo  include/gpk_thermo.hpp:229-263 */
+    GPK_MODEL_ANISO_OSTWALD  = 4   /* AnisoOstwaldThermo      include/gpk_thermo.hpp:284-325 */
+};
+
+/* flag bits of gpk_model_desc.flags (ThermoModel fields, include/gpk_core.hpp:72-123) */
+#define GPK_FLAG_GRAIN_NETWORK    0x1u
+#define GPK_FLAG_SOLUTE_REJECTION 0x2u
+#define GPK_FLAG_STOCHASTIC       0x4u
+
+typedef struct gpk_model_desc {
+    int32_t  kind;          /* gpk_model_kind */
+    uint32_t flags;         /* GPK_FLAG_* */
+    float sigma0;           /* baseline interfacial energy */
+    float eps4;             /* 4-fold / cubic anisotropy */
+    float eps6;             /* 6-fold 2-D anisotropy (extension; 0 = reference law) */
+    float M0;               /* mobility */
+    float k_partition;      /* solute partition coefficient (solute_rejection) */
+    float noise_scale;      /* stochastic kinetics amplitude */
+    float kB;               /* Boltzmann constant used by the noise term */
+    /* SolidificationThermo */
+    float m_liq, k_part, T_melt, L;
+    /* TensorAnisotropyThermo / AnisoOstwaldThermo */
+    float bxx, byy, bzz;
+    float c_sat, Omega;     /* AnisoOstwaldThermo */
+    float dG_bias;          /* TensorAnisotropyThermo (mutable between steps) */
+} gpk_model_desc;
+
+/* Fills *m with the defaults of the reference ThermoModel base and the given
+ * kind (gpk_core.hpp:73-123 and each model's constructor defaults). */
+void gpk_model_defaults(int32_t kind, gpk_model_desc* m);
+
+typedef struct gpk_grid_desc {
+    int32_t Nz, Ny, Nx;     /* Nz == 1 selects the 2-D path */
+    float   dx;             /* cell size */
+    float   D;              /* solute diffusivity */
+    float   gaussian_sigma; /* geometry Gaussian width in cells */
+    int32_t precision;      /* gpk_precision */
+    int32_t device;         /* HIP device ordinal */
+} gpk_grid_desc;
+
+/* Per-step counters (include/gpk_core.hpp:233-235 plus the band size). */
+typedef struct gpk_step_stats {
+    int64_t band_cells;     /* |A| after the mask pass at the start of the step */
+    int64_t candidates;     /* cells whose alpha crossed the flip threshold */
+    int64_t accepted;       /* topology_accepted_last */
+    int64_t deferred;       /* topology_deferred_last */
+    int64_t exchanges;      /* gaussian_remap_exchanges_last */
+    int64_t eroded;         /* cells relabelled by unresolved-grain extinction */
+    double  dt;             /* timestep used */
+    double  time;           /* simulation time after the step */
+    int64_t step_id;        /* steps taken by this handle */
+} gpk_step_stats;
+
+typedef struct gpk_handle gpk_handle;
+
+/* Thread-local text of the last error on this thread ("" if none). */
+const char* gpk_last_error(void);
+
+/* Library version string. */
+const char* gpk_version(void);
+
+/* Replaces GPKGrid::GPKGrid (include/gpk_core.hpp:237-240, src/gpk_grid.cpp:9-49):
+ * allocates the device state with the reference's initial values
+ * (phi 0, alpha 1, c 0, T 800, g 0, rho 0). */
+int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model,
+               gpk_handle** out);
+
+/* Releases every device buffer of the handle. */
+int gpk_destroy(gpk_handle* h);
+
+/* Replaces direct writes to the public GPKGrid vectors
+ * (include/gpk_core.hpp:212-228): copies a full field from host memory.
+ * bytes must equal N * sizeof(dtype).  Uploading PHI rebuilds the label
+ * tables (counts) used by the flip and extinction passes. */
+int gpk_upload(gpk_handle* h, int field, const void* host, size_t bytes);
+
+/* Replaces reads of the public GPKGrid vectors: copies a full field to host
+ * memory after the handle's stream has drained.  Band-only fields are
+ * returned at full size with zeros outside the band, as the reference
+ * holds them. */
+int gpk_download(gpk_handle* h, int field, void* host, size_t bytes);
+
+/* Device-to-device variants (for callers that keep state in HBM, e.g. torch):
+ * ptr is a device address on the handle's device; the copy is ordered on the
+ * handle's stream. */
+int gpk_upload_device(gpk_handle* h, int field, const void* dev_ptr, size_t bytes);
+int gpk_download_device(gpk_handle* h, int field, void* dev_ptr, size_t bytes);
+
+/* Replaces GPKGrid::step (include/gpk_core.hpp:243, src/gpk_simulation.cpp:31-54).
+ * dt < 0 selects the stable timestep.  Writes the timestep used to *dt_used
+ * and the step counters to *stats (either may be NULL). */
+int gpk_step(gpk_handle* h, float dt, float dt_cap, double* dt_used,
+             gpk_step_stats* stats);
+
+/* Runs nsteps consecutive step(dt, dt_cap) calls without returning to the
+ * host in between (no per-step synchronisation).  If stats is non-NULL it
+ * receives the last step's counters with band_cells, candidates, accepted,
+ * deferred, exchanges and eroded summed over the nsteps. */
+int gpk_step_many(gpk_handle* h, int nsteps, float dt, float dt_cap,
+                  gpk_step_stats* stats);
+
+/* Replaces GPKGrid::run (include/gpk_core.hpp:246-248, src/gpk_simulation.cpp:60-72)
+ * without the callback: steps until time >= t_end; *steps_taken receives the
+ * count.  Callbacks are driven by the host facade between calls. */
+int gpk_run(gpk_handle* h, double t_end, float dt, int* steps_taken);
+
+/* Updates the model parameters between steps (e.g. TensorAnisotropyThermo::
+ * dG_bias, include/gpk_thermo.hpp:238).  The kind must not change. */
+int gpk_set_model(gpk_handle* h, const gpk_model_desc* model);
+
+/* GPKGrid::time accessors (include/gpk_core.hpp:209). */
+int gpk_get_time(gpk_handle* h, double* t);
+int gpk_set_time(gpk_handle* h, double t);
+
+/* Diagnostics (src/gpk_simulation.cpp:78-123, include/gpk_core.hpp:250-255). */
+int gpk_max_dt(gpk_handle* h, double* dt);
+int gpk_phase_volume(gpk_handle* h, int32_t label, double* volume);
+int gpk_n_grains(gpk_handle* h, int* n);
+int gpk_mean_grain_radius(gpk_handle* h, double* radius);
+int gpk_solid_fraction(gpk_handle* h, float* fraction);
+int gpk_interface_fraction(gpk_handle* h, float* fraction);
+/* Sum of c over the grid in FP64 (fixed order, deterministic); solute mass. */
+int gpk_solute_sum(gpk_handle* h, double* sum);
+/* Bytes of device memory held by the handle (cf. allocated_array_bytes,
+ * src/gpk_simulation.cpp:125-144). */
+int gpk_device_bytes(gpk_handle* h, size_t* bytes);
+
+/* Replaces GPKGrid::topology_simple_move (include/gpk_core.hpp:264,
+ * src/gpk_connectivity.cpp:197-206): *result = 1 when relabelling cell to
+ * new_label passes the face-connectivity certificate on the current labels. */
+int gpk_topology_simple_move(gpk_handle* h, int cell, int32_t new_label, int* result);
+
+/* Replaces voronoi_init (demos/gpk.cpp:51-189): periodic Voronoi labels from
+ * n_seeds unique sites drawn with std::mt19937(seed), orientations from
+ * uniform_real<float>[0, pi/2), alpha = 1, then the face-connectivity
+ * normalisation.  Writes PHI, G, ALPHA on the device. */
+int gpk_voronoi_init(gpk_handle* h, int n_seeds, uint32_t seed);
+
+/* Host-only form of the same generator: fills phi[N] and g[N] (float). */
+int gpk_voronoi_host(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed,
+                     int32_t* phi, float* g);
+
+/* Blocks until the handle's stream has drained. */
+int gpk_sync(gpk_handle* h);
+
+/* Kernel timing of the last gpk_step_many call: average duration in ms of
+ * the geometry kernel launches (HIP events on the handle's stream) and the
+ * number of launches measured.  Enabled by gpk_set_profiling(h, 1). */
+int gpk_set_profiling(gpk_handle* h, int enable);
+int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launches,
+                      double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GPK_C_H */
