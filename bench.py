@@ -1,0 +1,206 @@
+"""Benchmark: interface cell-updates/s of 3-D isotropic grain growth on one MI355X.
+
+Workload: 256^3 periodic grid, Voronoi labels from 20,000 sites (seed 42),
+IsotropicGrowthThermo(M0 = 1, sigma0 = 1), D = 0, geometry sigma = 1.5 cells,
+dt at the stable limit.  A step is one GPKGrid::step of the whole grid.
+
+The headline `value` is the FP64 GPU path: sum over the K timed steps of the
+band size |A_s| divided by the wall time of those steps, with the state
+already resident in HBM.  The FP32 path is timed the same way and reported
+beside it, with the curvature error against FP64 on identical labels and the
+label disagreement after the run.  The CPU baseline times the reference step
+(oracle/_ref, built from the reference sources) on this host's cores for a
+bounded sample of the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "strict32": 157.3}
+
+
+def run_gpu(prec, size, grains, sigma, warmup, steps):
+    import gpk_amd
+    from gpk_amd import _abi
+
+    model = _abi.isotropic(1.0, 1.0)
+    G = gpk_amd.GPKGrid(size, size, size, 1.0, model, 0.0, sigma, precision=prec)
+    t0 = time.time()
+    G.voronoi_init(grains, 42)
+    G.sync()
+    t_init = time.time() - t0
+    for _ in range(warmup):
+        G.step()
+    G.sync()
+    G.set_profiling(True)
+    t0 = time.perf_counter()
+    st = G.step_many(steps)
+    G.sync()
+    wall = time.perf_counter() - t0
+    geo_ms, launches, flops = G.kernel_timing(0)
+    G.set_profiling(False)
+    return G, {
+        "precision": prec,
+        "steps": steps,
+        "warmup": warmup,
+        "band_cells_total": int(st.band_cells),
+        "band_fraction": float(st.band_cells) / (steps * size ** 3),
+        "seconds": wall,
+        "ms_per_step": 1e3 * wall / steps,
+        "cell_updates_per_s": st.band_cells / wall,
+        "accepted": int(st.accepted),
+        "deferred": int(st.deferred),
+        "exchanges": int(st.exchanges),
+        "geometry_ms_avg": geo_ms,
+        "geometry_launches": launches,
+        "geometry_flops_per_launch": flops,
+        "init_seconds": t_init,
+    }
+
+
+def kappa_error(size, grains, sigma):
+    """FP32 vs FP64 curvature on identical labels (the initial Voronoi map)."""
+    import gpk_amd
+    from gpk_amd import _abi
+
+    model = _abi.isotropic(1.0, 1.0)
+    G64 = gpk_amd.GPKGrid(size, size, size, 1.0, model, 0.0, sigma, precision="fp64")
+    G64.voronoi_init(grains, 42)
+    phi = G64.get("phi")
+    G32 = gpk_amd.GPKGrid(size, size, size, 1.0, model, 0.0, sigma, precision="fp32")
+    G32.set("phi", phi)
+    G64.step(0.0)
+    G32.step(0.0)
+    k64 = G64.get("kappa").ravel()
+    k32 = G32.get("kappa").ravel().astype(np.float64)
+    band = G64.get("mask").ravel().astype(bool)
+    floor = 1e-3 / sigma
+    rel = np.abs(k32[band] - k64[band]) / np.maximum(np.abs(k64[band]), floor)
+    G64.close()
+    G32.close()
+    return {"kappa_rel_err_max": float(rel.max()), "kappa_rel_err_rms": float(np.sqrt(np.mean(rel ** 2))),
+            "kappa_floor": floor}
+
+
+def band_size(phi):
+    """|A|: cells with a periodic face neighbour of another label (src/gpk_geometry.cpp:42-82)."""
+    m = np.zeros(phi.shape, bool)
+    for ax in range(phi.ndim):
+        for sh in (1, -1):
+            m |= phi != np.roll(phi, sh, axis=ax)
+    return int(np.count_nonzero(m))
+
+
+def cpu_baseline(phi, g, size, sigma, cpu_steps):
+    """Reference CPU step (compiled reference sources) on this host."""
+    from gpk_amd import _abi
+    from oracle import oracle as O
+
+    model = _abi.isotropic(1.0, 1.0)
+    kind = "reference" if O.ref_available() else "port"
+    if kind == "reference":
+        R = O.RefGrid(size, size, size, 1.0, model, 0.0, sigma)
+    else:
+        R = O.OracleGrid(size, size, size, 1.0, model, 0.0, sigma, "strict32")
+    R.set("phi", phi)
+    R.set("g", g)
+    band = 0
+    wall = 0.0
+    for _ in range(cpu_steps):
+        band += band_size(R.get("phi"))
+        t0 = time.perf_counter()
+        R.step()
+        wall += time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": band / wall, "unit": "interface cell-updates/s", "cores": threads,
+            "kind": kind, "seconds": wall,
+            "sample": "%d reference step(s) of the same 256^3 Voronoi state from step 0" % cpu_steps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--grains", type=int, default=20000)
+    ap.add_argument("--sigma", type=float, default=1.5)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the per-step kinetics path is built and measured "
+              "on a single MI355X (multi-GPU sharding is a design note, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    G64, r64 = run_gpu("fp64", args.size, args.grains, args.sigma, args.warmup, args.steps)
+    phi0 = None
+    res32 = None
+    if not args.no_fp32:
+        G32, res32 = run_gpu("fp32", args.size, args.grains, args.sigma, args.warmup, args.steps)
+        p64 = G64.get("phi")
+        p32 = G32.get("phi")
+        res32["label_disagreement"] = float(np.count_nonzero(p64 != p32)) / p64.size
+        G32.close()
+        res32.update(kappa_error(args.size, args.grains, args.sigma))
+        res32["speedup_vs_fp64"] = res32["cell_updates_per_s"] / r64["cell_updates_per_s"]
+    G64.close()
+
+    cpu = None
+    if not args.no_cpu:
+        from gpk_amd import voronoi_host
+        phi0, g0 = voronoi_host(args.size, args.size, args.size, args.grains, 42)
+        cpu = cpu_baseline(phi0, g0, args.size, args.sigma, args.cpu_steps)
+
+    flops = r64["geometry_flops_per_launch"]
+    ach = flops / (r64["geometry_ms_avg"] * 1e-3) / 1e12 if r64["geometry_ms_avg"] > 0 else 0.0
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "geometry_fp64_256_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "interface cell-updates/s, 256^3 grain growth: FP32 vs FP64 GPU vs CPU",
+        "value": r64["cell_updates_per_s"],
+        "unit": "interface cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (periodic Voronoi, 20000 sites, seed 42)",
+        "config": {"workload": "3-D isotropic grain growth, %d^3 periodic grid, %d Voronoi grains, "
+                               "sigma %.2f cells, FP64" % (args.size, args.grains, args.sigma),
+                   "grid": [args.size] * 3, "grains": args.grains, "sigma_cells": args.sigma,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "compute": "VALU (vector FP64; the stencil is not a contraction)",
+                     "kernel": "k_geometry<double,3,separable>",
+                     "achieved": ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": ach / PEAK_TFLOPS["fp64"], "traffic": traffic,
+                     "flops_per_launch": flops, "avg_launch_ms": r64["geometry_ms_avg"]},
+        "cpu_baseline": cpu,
+        "fp64": r64,
+        "fp32": res32,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = r64["cell_updates_per_s"] / cpu["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -53,6 +53,13 @@ inline int gsblk(long long n, int b) { return std::max(1, std::min(nblk(n, b), 8
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void swap(DevBuf& o) noexcept {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
     void alloc(size_t b) {
         release();
         if (b == 0) b = 16;
@@ -197,35 +204,39 @@ Sites draw_sites(int Nz, int Ny, int Nx, int n, uint32_t seed) {
     return S;
 }
 
-// host nearest-site search over a bucket grid; exact (all sites at the
-// minimal distance are inspected, lowest index wins)
+// host nearest-site search over a bucket grid; exact: a site in a bucket at
+// periodic bucket distance k >= ring+1 along some axis lies at least
+// ring*Bmin + 1 cells away along that axis, so the search stops once the best
+// squared distance is below that bound (ties at the bound keep searching and
+// the lowest site index wins, as in the reference's strict < scan)
 void nearest_host(int Nz, int Ny, int Nx, const Sites& S, int32_t* phi) {
     const int n = static_cast<int>(S.z.size());
     const long long N = 1LL * Nz * Ny * Nx;
     const double per = std::max(1.0, double(N) / n);
-    int B = std::max(1, static_cast<int>(std::cbrt(per)));
-    if (Nz == 1) B = std::max(1, static_cast<int>(std::sqrt(per)));
-    const int bz = Nz == 1 ? 1 : std::max(1, (Nz + B - 1) / B);
-    const int by = std::max(1, (Ny + B - 1) / B), bx = std::max(1, (Nx + B - 1) / B);
-    const int Bz = Nz == 1 ? 1 : B;
+    const int B = std::max(1, Nz == 1 ? static_cast<int>(std::sqrt(per)) : static_cast<int>(std::cbrt(per)));
+    const int Bz = Nz == 1 ? Nz : std::min(B, Nz), By = std::min(B, Ny), Bx = std::min(B, Nx);
+    const int bz = (Nz + Bz - 1) / Bz, by = (Ny + By - 1) / By, bx = (Nx + Bx - 1) / Bx;
+    // smallest bucket extent over the axes that are searched
+    int Bmin = std::min(Ny - (by - 1) * By, By);
+    Bmin = std::min(Bmin, std::min(Nx - (bx - 1) * Bx, Bx));
+    if (Nz > 1) Bmin = std::min(Bmin, std::min(Nz - (bz - 1) * Bz, Bz));
+    Bmin = std::max(Bmin, 1);
     std::vector<std::vector<int>> bucket(size_t(bz) * by * bx);
     for (int k = 0; k < n; ++k)
-        bucket[(size_t(S.z[k] / Bz) * by + S.y[k] / B) * bx + S.x[k] / B].push_back(k);
+        bucket[(size_t(S.z[k] / Bz) * by + S.y[k] / By) * bx + S.x[k] / Bx].push_back(k);
     auto worker = [&](int z0, int z1) {
         for (int z = z0; z < z1; ++z)
             for (int y = 0; y < Ny; ++y)
                 for (int x = 0; x < Nx; ++x) {
                     long long bd = LLONG_MAX;
                     int best = 0;
-                    const int cz = z / Bz, cy = y / B, cx = x / B;
+                    const int cz = z / Bz, cy = y / By, cx = x / Bx;
                     for (int ring = 0;; ++ring) {
-                        // ring of buckets at Chebyshev bucket distance `ring`
                         const int rz = Nz == 1 ? 0 : ring;
                         for (int dz = -rz; dz <= rz; ++dz)
                             for (int dy = -ring; dy <= ring; ++dy)
                                 for (int dxb = -ring; dxb <= ring; ++dxb) {
                                     if (std::max({std::abs(dz), std::abs(dy), std::abs(dxb)}) != ring) continue;
-                                    if (2 * std::abs(dz) + 1 > bz + 1 && bz > 1 && std::abs(dz) * 2 > bz) continue;
                                     const int qz = ((cz + dz) % bz + bz) % bz;
                                     const int qy = ((cy + dy) % by + by) % by;
                                     const int qx = ((cx + dxb) % bx + bx) % bx;
@@ -239,21 +250,19 @@ void nearest_host(int Nz, int Ny, int Nx, const Sites& S, int32_t* phi) {
                                         if (d < bd || (d == bd && k < best)) { bd = d; best = k; }
                                     }
                                 }
-                        // every site outside rings 0..ring lies at a periodic
-                        // distance of at least ring*B cells along some axis
-                        const long long reach = 1LL * ring * B;
                         const bool covered_all = (2 * ring + 1 >= bx) && (2 * ring + 1 >= by) &&
                                                  (Nz == 1 || 2 * ring + 1 >= bz);
                         if (covered_all) break;
-                        if (bd != LLONG_MAX && reach * reach > bd) break;
+                        const long long lb = 1LL * ring * Bmin + 1;
+                        if (bd < lb * lb) break;
                     }
                     phi[(size_t(z) * Ny + y) * Nx + x] = best + 1;
                 }
     };
     const int nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    const int step = (Nz + nt - 1) / nt;
     if (Nz >= nt) {
+        std::vector<std::thread> th;
+        const int step = (Nz + nt - 1) / nt;
         for (int t = 0; t < nt; ++t) th.emplace_back(worker, t * step, std::min(Nz, (t + 1) * step));
         for (auto& t : th) t.join();
     } else {
@@ -400,12 +409,16 @@ struct Engine final : EngineBase {
     int geo_band = 0;           // size of the geometry band (bandA)
     bool fields_valid = false;
 
-    // profiling
+    // profiling: one event pair per geometry launch, read after the run so
+    // the timed steps are not synchronised by the measurement
     bool profiling = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evpool;
+    size_t evused = 0;
     double geo_ms = 0.0;
     int64_t geo_launches = 0;
-    double geo_bytes = 0.0;
+    double geo_flops = 0.0;     // algorithmic flops of the measured launches
+    double geo_band_sum = 0.0;
 
     Engine(const gpk_grid_desc& gd, const gpk_model_desc& m)
         : Nz(gd.Nz), Ny(gd.Ny), Nx(gd.Nx), N(gd.Nz * gd.Ny * gd.Nx), dim3(gd.Nz > 1 ? 1 : 0),
@@ -425,6 +438,7 @@ struct Engine final : EngineBase {
 
     ~Engine() override {
         if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& e : evpool) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         if (hs) (void)hipHostFree(hs);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -673,14 +687,48 @@ struct Engine final : EngineBase {
     void sync() override { HIPCHK(hipStreamSynchronize(stream)); }
     void set_profiling(int on) override {
         profiling = on != 0;
+        evused = 0;
         geo_ms = 0.0;
         geo_launches = 0;
+        geo_flops = 0.0;
+        geo_band_sum = 0.0;
     }
+    void record_geo_begin() {
+        if (evused == evpool.size()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            evpool.emplace_back(a, b);
+        }
+        HIPCHK(hipEventRecord(evpool[evused].first, stream));
+    }
+    void record_geo_end(int nb) {
+        HIPCHK(hipEventRecord(evpool[evused].second, stream));
+        ++evused;
+        // reference-formulation flops of the geometry stencil per band cell:
+        // 2 flops (multiply-add) per window cell and accumulated component
+        const double W = 2.0 * W_rg() + 1.0;
+        const double per = dim3 ? 2.0 * W * W * W * 9.0 : 2.0 * W * W * 5.0;
+        geo_flops += per * nb;
+        geo_band_sum += nb;
+    }
+    double W_rg() const { return double(W.rg); }
+    // which = 0: geometry kernel.  avg_ms, launches, and (through bytes) the
+    // algorithmic flops per launch of the geometry stencil
     void timing(int which, double* ms, int64_t* n, double* bytes) override {
         (void)which;
+        HIPCHK(hipStreamSynchronize(stream));
+        double tot = 0.0;
+        for (size_t e = 0; e < evused; ++e) {
+            float v = 0.f;
+            HIPCHK(hipEventElapsedTime(&v, evpool[e].first, evpool[e].second));
+            tot += v;
+        }
+        geo_ms = tot;
+        geo_launches = int64_t(evused);
         if (ms) *ms = geo_launches ? geo_ms / double(geo_launches) : 0.0;
         if (n) *n = geo_launches;
-        if (bytes) *bytes = geo_launches ? geo_bytes / double(geo_launches) : 0.0;
+        if (bytes) *bytes = geo_launches ? geo_flops / double(geo_launches) : 0.0;
     }
 
     // ---- band compaction: mask + sorted interface list (+ slot map) ----
@@ -713,7 +761,7 @@ struct Engine final : EngineBase {
         // geometry + competitor + constitutive velocity
         GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
         if (nb > 0) {
-            if (profiling) HIPCHK(hipEventRecord(ev0, stream));
+            if (profiling) record_geo_begin();
             if (dim3)
                 k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                     dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
@@ -724,17 +772,7 @@ struct Engine final : EngineBase {
                     dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
                     wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
                     nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
-            if (profiling) {
-                HIPCHK(hipEventRecord(ev1, stream));
-                HIPCHK(hipEventSynchronize(ev1));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ev0, ev1));
-                geo_ms += ms;
-                ++geo_launches;
-                // compulsory state traffic of the step per launch: labels of the
-                // grid (4 B/cell) + alpha read/write and g read per band cell
-                geo_bytes += 4.0 * N + (2.0 * sizeof(R) + sizeof(R)) * nb;
-            }
+            if (profiling) record_geo_end(nb);
         }
         k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
         if (nb > 0) {
@@ -774,7 +812,7 @@ struct Engine final : EngineBase {
         } else if (D_f > 0.f) {
             k_diffusion<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, c.as<R>(), mask.as<uint8_t>(),
                                                               alpha.as<R>(), ds(), dx_f, D_f, work2.as<R>());
-            std::swap(c, work2);
+            c.swap(work2);
         }
         k_time_advance<<<1, 1, 0, stream>>>(ds());
         ++step_id;
@@ -784,7 +822,7 @@ struct Engine final : EngineBase {
             st->candidates = m;
             st->accepted = hs->accepted;
             st->deferred = hs->deferred;
-            st->exchanges = hs->exchanges;
+            st->exchanges = exchanges;
             st->eroded = hs->eroded;
             st->dt = hs->dt;
             st->time = hs->time;
@@ -974,7 +1012,7 @@ struct Engine final : EngineBase {
                                                                 c.as<R>(), work1.as<R>(), work2.as<R>());
             k_diffusion_part<R><<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, work1.as<R>(), work2.as<R>(),
                                                                    c.as<R>(), ds(), dx_f, D_f, work3.as<R>());
-            std::swap(c, work3);
+            c.swap(work3);
         }
     }
 

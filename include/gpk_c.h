@@ -222,12 +222,16 @@ int gpk_voronoi_host(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed,
 /* Blocks until the handle's stream has drained. */
 int gpk_sync(gpk_handle* h);
 
-/* Kernel timing of the last gpk_step_many call: average duration in ms of
- * the geometry kernel launches (HIP events on the handle's stream) and the
- * number of launches measured.  Enabled by gpk_set_profiling(h, 1). */
+/* Kernel timing (no reference counterpart; measurement support).
+ * gpk_set_profiling(h, 1) resets the counters and starts recording a HIP
+ * event pair around every geometry launch on the handle's stream (no host
+ * synchronisation is added).  gpk_kernel_timing(h, 0, ...) drains the stream
+ * and returns the average launch duration in ms, the number of launches and
+ * the algorithmic flops per launch of the reference-formulation stencil
+ * (2 * window cells * components * band cells). */
 int gpk_set_profiling(gpk_handle* h, int enable);
 int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launches,
-                      double* bytes_per_launch);
+                      double* flops_per_launch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,122 @@
+"""Writes the committed golden fixtures of tests/golden from the reference tree.
+
+Run in a container that has the reference mounted (default /root/reference)
+and the reference checker built (make -C oracle):
+
+    python tests/golden/make_fixtures.py [--ref /root/reference]
+
+Fixtures (inputs and expected outputs only):
+  t1_curvature.npz   weighted-mean curvature of digitised circles R = 10..60,
+                     sigma = 2R/15, from the reference's committed
+                     paper/data/validate_theory_foundations/t1_curvature_error.csv
+                     (demos/validate_theory_foundations.cpp:107-158)
+  g13.npz g14.npz g15.npz
+                     curved-bicrystal label fields before/after 800 steps and the
+                     (time, area) trajectory, from the reference's committed
+                     paper/data/validate_extended_base_models/g1{3,4,5}_*.csv
+                     (demos/validate_extended_base_models.cpp:757-830)
+  voronoi_*.npz      initial labels and orientations written by the reference's
+                     own CLI (demos/gpk.cpp voronoi_init, 51-189) for small decks,
+                     run from oracle/_ref/gpk
+"""
+import argparse
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+
+
+def read_grid_csv(path):
+    """Label field of write_grid_csv (two header lines, one label per line)."""
+    with open(path) as f:
+        head = f.readline()
+        f.readline()
+        vals = np.array([int(line) for line in f], dtype=np.int32)
+    ny = int(head.split("Ny=")[1].split()[0])
+    nx = int(head.split("Nx=")[1].split()[0])
+    return vals.reshape(ny, nx)
+
+
+def t1(ref):
+    src = os.path.join(ref, "paper/data/validate_theory_foundations/t1_curvature_error.csv")
+    rows = [line.strip().split(",") for line in open(src).read().splitlines()[1:] if line.strip()]
+    R = np.array([int(float(r[0])) for r in rows], np.int32)
+    # keep the printed text: the fixture holds 9 significant digits (%.9g)
+    kappa_text = np.array([r[1] for r in rows])
+    np.savez_compressed(os.path.join(HERE, "t1_curvature.npz"), R=R, kappa_text=kappa_text)
+
+
+def bicrystal(ref, key):
+    d = os.path.join(ref, "paper/data/validate_extended_base_models")
+    before = read_grid_csv(os.path.join(d, key + "_grid_before.csv"))
+    after = read_grid_csv(os.path.join(d, key + "_grid_after.csv"))
+    rows = [line.strip().split(",") for line in
+            open(os.path.join(d, key + "_area_t.csv")).read().splitlines()[1:] if line.strip()]
+    assert before.max() <= 1 and after.max() <= 1
+    np.savez_compressed(os.path.join(HERE, key + ".npz"),
+                        shape=np.array(before.shape, np.int32),
+                        before=np.packbits(before.astype(np.uint8).ravel()),
+                        after=np.packbits(after.astype(np.uint8).ravel()),
+                        t_text=np.array([r[0] for r in rows]),
+                        area_text=np.array([r[1] for r in rows]))
+
+
+DECK = """[run]
+model = grain_growth
+[grid]
+Nz = {nz}
+Ny = {ny}
+Nx = {nx}
+dx = 1.0e-6
+[time]
+t_end = 1e-12
+n_snaps = 1
+[thermo]
+M0 = 1.0e-8
+sigma0 = 0.5
+[grain_growth]
+n_grains = {ng}
+seed = {seed}
+[geometry]
+sigma_cells = 1.5
+[output]
+format = csv
+dir = {out}
+write_initial = true
+"""
+
+
+def voronoi(cli, nz, ny, nx, ng, seed):
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = os.path.join(tmp, "deck.ini")
+        out = os.path.join(tmp, "out")
+        with open(deck, "w") as f:
+            f.write(DECK.format(nz=nz, ny=ny, nx=nx, ng=ng, seed=seed, out=out))
+        subprocess.check_call([cli, deck], stdout=subprocess.DEVNULL, cwd=tmp)
+        rows = np.loadtxt(os.path.join(out, "snap_0000.csv"), comments="#")
+    # CSV rows are written z-major, then y, then x (utils/csv_writer.hpp:44-56)
+    phi = rows[:, 3].astype(np.int32).reshape(nz, ny, nx)
+    g_text = np.array(["%.6f" % v for v in rows[:, 7]])
+    name = "voronoi_%dx%dx%d_%d_%d.npz" % (nz, ny, nx, ng, seed)
+    np.savez_compressed(os.path.join(HERE, name), phi=phi, g_text=g_text,
+                        dims=np.array([nz, ny, nx, ng, seed], np.int64))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    args = ap.parse_args()
+    t1(args.ref)
+    for key in ("g13", "g14", "g15"):
+        bicrystal(args.ref, key)
+    cli = os.path.join(REPO, "oracle", "_ref", "gpk")
+    for dims in ((1, 64, 48, 30, 7), (1, 96, 96, 200, 42), (16, 20, 24, 40, 3), (24, 24, 24, 60, 42)):
+        voronoi(cli, *dims)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,123 @@
+"""GPU parity: every precision mode of the HIP path against the same mode of
+the oracle restatement, bit for bit, step by step (labels, alpha, g, c, mask,
+phi_comp, normals, curvature, velocity, dt and the transaction counters).
+
+strict32 is the reference's arithmetic (the oracle's strict32 mode is pinned
+to the compiled reference and its fixtures by the CPU tests); fp32 and fp64
+use the separable geometry order defined by the oracle's fp32/fp64 modes.
+"""
+import numpy as np
+import pytest
+
+import gpk_amd
+from gpk_amd import _abi
+from oracle import oracle as O
+from tests.util import FIELDS_ALL, solidification_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def lockstep(o, G, steps, fields=FIELDS_ALL, check_every=1):
+    for s in range(steps):
+        d1 = o.step()
+        d2 = G.step()
+        assert d1 == d2, ("dt", s, d1, d2)
+        st = o.stats()
+        gs = G.last_stats
+        assert st["band"] == gs.band_cells, ("band", s)
+        assert (st["candidates"], st["accepted"], st["deferred"], st["exchanges"], st["eroded"]) == \
+            (gs.candidates, gs.accepted, gs.deferred, gs.exchanges, gs.eroded), ("counters", s)
+        if s % check_every == 0 or s == steps - 1:
+            for f in fields:
+                a, b = o.get(f), G.get(f)
+                assert np.array_equal(a, b), (f, s, int(np.count_nonzero(a != b)))
+    assert o.time == G.time
+
+
+def pair(dims, model, D, sigma, mode, init, dx=1.0):
+    o = O.OracleGrid(*dims, dx, model, D, sigma, mode)
+    G = gpk_amd.GPKGrid(*dims, dx, model, D, sigma, precision=mode)
+    for name, arr in init.items():
+        o.set(name, np.asarray(arr).astype(o.real) if name not in ("phi",) else arr)
+        G.set(name, np.asarray(arr).astype(G.real) if name not in ("phi",) else arr)
+    return o, G
+
+
+def test_config1_grain_growth_2d_strict32_100_steps():
+    # 256^2, 200 Voronoi grains (seed 42), sigma 1.5, dt at the stability limit
+    phi, g = O.voronoi(1, 256, 256, 200, 42)
+    o, G = pair((1, 256, 256), _abi.isotropic(1.0, 1.0), 0.0, 1.5, "strict32", {"phi": phi, "g": g})
+    lockstep(o, G, 100)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_config1_grain_growth_2d_fast_modes(mode):
+    phi, g = O.voronoi(1, 256, 256, 200, 42)
+    o, G = pair((1, 256, 256), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 60, check_every=5)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_grain_growth_3d(mode):
+    # 48^3 at the 256^3 / 20000-grain density (~839 cells per grain)
+    phi, g = O.voronoi(48, 48, 48, 132, 42)
+    o, G = pair((48, 48, 48), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 40 if mode == "strict32" else 25, check_every=4)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_solidification_partitioned_3d(mode):
+    model, phi, c, T = solidification_setup(32)
+    o, G = pair((32, 32, 32), model, 1e-9, 1.5, mode, {"phi": phi, "c": c, "T": T}, dx=1e-6)
+    lockstep(o, G, 40, check_every=4)
+    assert o.solute_sum() == G.solute_sum()
+
+
+def test_grain_growth_sigma2_3d_strict32():
+    phi, g = O.voronoi(32, 32, 32, 40, 9)
+    o, G = pair((32, 32, 32), _abi.isotropic(1.0, 1.0), 0.0, 2.0, "strict32", {"phi": phi, "g": g})
+    lockstep(o, G, 24, check_every=3)
+
+
+def test_aniso_ostwald_lamella_2d_strict32():
+    model = _abi.aniso_ostwald(1e-8, 0.5, 1.0, 4.0, 1.0, 0.1, 1e8)
+    phi = np.zeros((96, 128), np.int32)
+    phi[40:52, :] = 1
+    phi[39, ::9] = 1
+    phi[52, 4::11] = 1
+    c = np.full(phi.shape, 0.1, np.float32)
+    o, G = pair((1, 96, 128), model, 0.0, 1.5, "strict32", {"phi": phi, "c": c}, dx=1e-6)
+    lockstep(o, G, 60, check_every=5)
+
+
+def test_tensor_aniso_with_bias_2d_strict32():
+    model = _abi.tensor_aniso(1e-8, 0.5, 1.0, 2.0, 1.0, dG_bias=2e4)
+    y, x = np.mgrid[0:80, 0:80]
+    phi = ((y - 40) ** 2 / 225.0 + (x - 40) ** 2 / 100.0 <= 1).astype(np.int32)
+    o, G = pair((1, 80, 80), model, 0.0, 1.5, "strict32", {"phi": phi}, dx=1e-6)
+    lockstep(o, G, 40, check_every=4)
+
+
+def test_plain_diffusion_2d_strict32():
+    model = _abi.solidification(M0=5e-8, sigma0=0.2, eps4=0.0)
+    y, x = np.mgrid[0:64, 0:64]
+    phi = ((y - 32) ** 2 + (x - 32) ** 2 <= 36).astype(np.int32)
+    c = np.full((64, 64), 0.02, np.float32)
+    T = np.full((64, 64), 903.0, np.float32)
+    o, G = pair((1, 64, 64), model, 1e-9, 1.5, "strict32", {"phi": phi, "c": c, "T": T}, dx=1e-6)
+    lockstep(o, G, 40, check_every=4)
+
+
+def test_anisotropic_growth_2d_labels_and_tolerance():
+    # the 2-D cos(4 theta) law calls cosf/atan2f: device and host libm may
+    # differ in the last place, so velocities are compared to 1e-5 relative
+    # and the labels, band and counters must still agree exactly
+    phi, g = O.voronoi(1, 128, 128, 60, 42)
+    o, G = pair((1, 128, 128), _abi.anisotropic_growth(1.0, 1.0, 0.05), 0.0, 1.5, "strict32",
+                {"phi": phi, "g": g})
+    for s in range(30):
+        o.step()
+        G.step()
+        assert np.array_equal(o.get("phi"), G.get("phi")), s
+        vo, vg = o.get("vn").astype(np.float64), G.get("vn").astype(np.float64)
+        assert np.all(np.abs(vo - vg) <= 1e-5 * np.maximum(np.abs(vo).max(), 1e-30)), s

@@ -1,0 +1,70 @@
+"""Shared set-ups for the parity tests (initial conditions of the configs)."""
+import os
+
+import numpy as np
+
+from gpk_amd import _abi
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+FIELDS_ALL = ("phi", "alpha", "g", "mask", "phi_comp", "kappa", "nx", "ny", "nz", "vn", "g_comp", "c")
+
+
+def load_bicrystal(key):
+    z = np.load(os.path.join(GOLDEN, key + ".npz"))
+    ny, nx = (int(v) for v in z["shape"])
+    before = np.unpackbits(z["before"])[: ny * nx].reshape(ny, nx).astype(np.int32)
+    after = np.unpackbits(z["after"])[: ny * nx].reshape(ny, nx).astype(np.int32)
+    return before, after, list(z["t_text"]), list(z["area_text"])
+
+
+def bicrystal_shape(key, n=360):
+    y, x = np.mgrid[0:n, 0:n]
+    dy, dx = y - n // 2, x - n // 2
+    if key == "g13":
+        return (dy * dy + dx * dx <= 72 * 72).astype(np.int32)
+    if key == "g14":
+        return ((dy / 88.0) ** 2 + (dx / 48.0) ** 2 <= 1.0).astype(np.int32)
+    return ((np.abs(dy) <= 64) & (np.abs(dx) <= 64)).astype(np.int32)
+
+
+def circle(N, R):
+    y, x = np.mgrid[0:N, 0:N]
+    return ((y - N // 2) ** 2 + (x - N // 2) ** 2 <= R * R).astype(np.int32)
+
+
+def weighted_kappa(mask, phi, nx, ny, kappa):
+    """Weighted mean curvature of the T1 check (demos/validate_theory_foundations.cpp:126-135)."""
+    sel = np.flatnonzero((mask.astype(bool) & (phi == 1)).ravel())
+    nxv = nx.ravel()[sel].astype(np.float64)
+    nyv = ny.ravel()[sel].astype(np.float64)
+    dom = np.maximum(np.abs(nxv), np.abs(nyv))
+    cut = np.where(dom > 1e-8, 1.0 / np.where(dom > 1e-8, dom, 1.0), 1.0)
+    kv = kappa.ravel()[sel].astype(np.float64)
+    # sequential double sums in index order, as the reference loop
+    wk = 0.0
+    w = 0.0
+    for a, b in zip(kv, cut):
+        wk += a * b
+        w += b
+    return wk / w
+
+
+def solidification_setup(N, rng_seed=7, c0=0.02, k=0.14, undercooling=0.5, r=6):
+    """128^3-style solute-coupled solidification start: a solid sphere of radius r
+    at the centre with c = c0*k inside, c0 + U(-1e-3, 1e-3) outside; the
+    dimensionless undercooling Delta maps to dT = Delta*|m|c0(1-k)/k."""
+    m_liq, T_melt = -500.0, 933.0
+    model = _abi.solidification(M0=5e-8, sigma0=0.2, eps4=0.02, m_liq=m_liq, k_part=k,
+                                T_melt=T_melt, L=1e8, solute_rejection=True)
+    zz, yy, xx = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    c = N // 2
+    inside = (zz - c) ** 2 + (yy - c) ** 2 + (xx - c) ** 2 <= r * r
+    phi = np.zeros((N, N, N), np.int32)
+    phi[inside] = 1
+    rng = np.random.default_rng(rng_seed)
+    conc = (c0 + rng.uniform(-1e-3, 1e-3, size=phi.shape)).astype(np.float32)
+    conc[inside] = np.float32(c0 * k)
+    dT = undercooling * abs(m_liq) * c0 * (1 - k) / k
+    T = np.full(phi.shape, T_melt + m_liq * c0 - dT, np.float32)
+    return model, phi, conc, T
