@@ -175,6 +175,37 @@ std::vector<double> radial_table(float s, int r, bool single) {
     return t;
 }
 
+// Row-sum tables of the separable modes (same construction as the device
+// reads them, see RowLutDesc): nch = ceil(W/9) chunks of C = ceil(W/nch) dx,
+// entry base_k + idx*3 + a = sum of weight set a over the set bits of idx in
+// ascending dx (the most significant bit of idx is the chunk's first dx).
+template <class R>
+RowLutDesc make_row_lut(const std::vector<R>& w0, const std::vector<R>& w1, const std::vector<R>& w2,
+                        std::vector<R>& t) {
+    RowLutDesc L{};
+    L.W = static_cast<int>(w0.size());
+    L.nch = (L.W + 8) / 9;
+    const int C = (L.W + L.nch - 1) / L.nch;
+    int off = 0;
+    for (int k = 0; k < L.nch; ++k) {
+        L.start[k] = k * C;
+        L.len[k] = std::min(C, L.W - L.start[k]);
+        L.base[k] = off;
+        off += (1 << L.len[k]) * 3;
+    }
+    t.assign(off, R(0));
+    const std::vector<R>* w[3] = {&w0, &w1, &w2};
+    for (int k = 0; k < L.nch; ++k)
+        for (int idx = 0; idx < (1 << L.len[k]); ++idx)
+            for (int a = 0; a < 3; ++a) {
+                R sum = R(0);
+                for (int j = 0; j < L.len[k]; ++j)
+                    if ((idx >> (L.len[k] - 1 - j)) & 1) sum += (*w[a])[L.start[k] + j];
+                t[L.base[k] + idx * 3 + a] = sum;
+            }
+    return L;
+}
+
 // ---------------------------------------------------------------------------
 // Voronoi generator (demos/gpk.cpp:51-189).  Site draws use std::mt19937 and
 // the libstdc++ distributions; the nearest-site labels come from k_voronoi;
@@ -389,6 +420,7 @@ struct Engine final : EngineBase {
     float vel_sigma = 0.85f, remap_sigma = 0.7f;
     int64_t step_id = 0;
     int nlab = 0;           // label table size (max label + 1)
+    RowLutDesc lutd{};      // row tables of the separable modes
 
     // ---- device state ----
     DevBuf phi, mask, phi_comp, slot, cand_id, pid, mark;
@@ -540,7 +572,10 @@ struct Engine final : EngineBase {
             tab.alloc(t.size() * rs);
             HIPCHK(hipMemcpy(tab.p, t.data(), t.size() * rs, hipMemcpyHostToDevice));
         } else {
-            tab.alloc(16);
+            std::vector<R> t;
+            lutd = make_row_lut<R>(w0, w1, w2, t);
+            tab.alloc(t.size() * rs);
+            HIPCHK(hipMemcpy(tab.p, t.data(), t.size() * rs, hipMemcpyHostToDevice));
         }
         const bool single = sizeof(R) == 4;
         const auto pw = radial_table(vel_sigma, proj_r, single);
@@ -596,7 +631,7 @@ struct Engine final : EngineBase {
 
     void rebuild_counts() {
         fill<int32_t>(count, nlab, 0);
-        k_label_hist<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), count.as<int>());
+        k_label_hist<<<nblk((N + 63) / 64, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), count.as<int>());
     }
 
     void check_labels() {
@@ -731,6 +766,45 @@ struct Engine final : EngineBase {
         if (bytes) *bytes = geo_launches ? geo_flops / double(geo_launches) : 0.0;
     }
 
+    // ---- geometry launch: LDS-tiled separable kernel when the labels fit
+    // 16 bits and the radius has an instantiation, else the generic kernel
+    template <int DIM, int RG>
+    void launch_tiled(const GeomParams& gp) {
+        using S = TiledLds<R, DIM, RG>;
+        const int ntx = (Nx + S::TX - 1) / S::TX, nty = (Ny + S::TY - 1) / S::TY,
+                  ntz = (Nz + S::TZ - 1) / S::TZ;
+        const int ntiles = ntx * nty * ntz;
+        const int grid = 8 * ((ntiles + 7) / 8);
+        k_geometry_tiled<R, DIM, RG><<<grid, 256, 0, stream>>>(
+            dims, gp, model, ntx, nty, ntz, phi.as<int32_t>(), mask.as<uint8_t>(), slot.as<int>(),
+            wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(),
+            nx.as<R>(), ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(),
+            g_comp.as<R>());
+    }
+    template <int DIM>
+    bool try_tiled(const GeomParams& gp) {
+        switch (W.rg) {
+        case 6: launch_tiled<DIM, 6>(gp); return true;
+        case 8: launch_tiled<DIM, 8>(gp); return true;
+        case 9: launch_tiled<DIM, 9>(gp); return true;
+        case 12: launch_tiled<DIM, 12>(gp); return true;
+        default: return false;
+        }
+    }
+    void launch_geometry(int nb, const GeomParams& gp) {
+        if (!kDirect && nlab <= 65536 && (dim3 ? try_tiled<3>(gp) : try_tiled<2>(gp))) return;
+        if (dim3)
+            k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
+                dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+        else
+            k_geometry<R, 2, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
+                dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+    }
+
     // ---- band compaction: mask + sorted interface list (+ slot map) ----
     int band_compact(DevBuf& list, bool with_slot) {
         k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
@@ -762,16 +836,7 @@ struct Engine final : EngineBase {
         GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
         if (nb > 0) {
             if (profiling) record_geo_begin();
-            if (dim3)
-                k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
-                    dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                    wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
-                    nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
-            else
-                k_geometry<R, 2, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
-                    dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                    wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
-                    nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+            launch_geometry(nb, gp);
             if (profiling) record_geo_end(nb);
         }
         k_vmax_init<R><<<1, 1, 0, stream>>>(ds());

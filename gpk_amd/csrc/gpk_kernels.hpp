@@ -70,12 +70,125 @@ template <> struct Eps<double> {
 __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Row-sum tables of the separable modes: window-row "same label" bits m
+// (dx = 0 at the most significant of W bits) cut into nch chunks of at most C
+// consecutive dx; chunk k covers dx in [start_k, start_k + len_k) and its
+// table entry idx*3 + a holds the sum of weight set a over the set bits in
+// ascending dx.  The row sum adds the chunk values in ascending chunk order.
+struct RowLutDesc {
+    int W, nch;
+    int start[8], len[8], base[8];
+};
+
+template <class R>
+__device__ __forceinline__ void row_sum(const RowLutDesc& L, const R* __restrict__ t, uint64_t m,
+                                        R& R0, R& R1, R& R2) {
+    for (int k = 0; k < L.nch; ++k) {
+        const int sh = L.W - L.start[k] - L.len[k];
+        const int ix = int((m >> sh) & ((1ull << L.len[k]) - 1ull));
+        const R* e = t + L.base[k] + ix * 3;
+        if (k == 0) { R0 = e[0]; R1 = e[1]; R2 = e[2]; }
+        else { R0 += e[0]; R1 += e[1]; R2 += e[2]; }
+    }
+}
+
+template <class R>
+struct LutRef {
+    RowLutDesc L;
+    const R* t;
+};
+template <class R>
+__device__ __forceinline__ void row_sum(const LutRef<R>& lr, uint64_t m, R& R0, R& R1, R& R2) {
+    row_sum<R>(lr.L, lr.t, m, R0, R1, R2);
+}
+
+// normals, curvature (src/gpk_geometry.cpp:136-151), competitor (154-182) and
+// the constitutive velocity of band cell i (slot a) from its stencil sums
+template <class R, int DIM, class CubeAt>
+__device__ __forceinline__ void finish_cell(
+    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
+    int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
+    const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
+    const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
+    R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
+    R* __restrict__ g_comp) {
+    const R dx = R(gp.dx_f);
+    const R inv_dx = R(1) / dx;
+    const R inv_dx2 = inv_dx * inv_dx;
+    const R klim = R(1) / (R(gp.sigma_cells) * dx);
+    Ix *= inv_dx; Iy *= inv_dx; Iz *= inv_dx;
+    Ixx *= inv_dx2; Iyy *= inv_dx2; Izz *= inv_dx2;
+    Ixy *= inv_dx2; Ixz *= inv_dx2; Iyz *= inv_dx2;
+    const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
+    R nxv = 0, nyv = 0, nzv = 0, kap = 0;
+    if (gsq * dx * dx > Eps<R>::grad) {
+        const R gr = dsqrt(gsq);
+        nxv = -Ix / gr; nyv = -Iy / gr; nzv = -Iz / gr;
+        const R trH = Ixx + Iyy + Izz;
+        const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
+                    + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
+        R kv = (gHg - gsq * trH) / (gsq * gr);
+        if (dfabs(kv * dx) < Eps<R>::kappa) kv = R(0);
+        kap = kv < -klim ? -klim : (klim < kv ? klim : kv);
+    }
+    onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
+    (void)cube_at;
+
+    // competitor: most-supported unlike face label, ties by outward-normal
+    // alignment, then first face (src/gpk_geometry.cpp:154-182)
+    int f[6];
+    faces(d, i, f);
+    const R score[6] = {-nxv, nxv, -nyv, nyv, -nzv, nzv};
+    const int nface = DIM == 3 ? 6 : 4;
+    int32_t cube[27];
+    {
+        int t = 0;
+        const int zb = DIM == 3 ? -1 : 0, ze = DIM == 3 ? 1 : 0;
+        for (int dz = zb; dz <= ze; ++dz) {
+            int zz = z0 + dz; if (zz < 0) zz += d.Nz; else if (zz >= d.Nz) zz -= d.Nz;
+            for (int dy = -1; dy <= 1; ++dy) {
+                int yy = y0 + dy; if (yy < 0) yy += d.Ny; else if (yy >= d.Ny) yy -= d.Ny;
+                for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
+                    int xx = x0 + ddx; if (xx < 0) xx += d.Nx; else if (xx >= d.Nx) xx -= d.Nx;
+                    cube[t] = cube_at(t, zz, yy, xx);
+                }
+            }
+        }
+    }
+    const int ncube = DIM == 3 ? 27 : 9, ccentre = DIM == 3 ? 13 : 4;
+    int best = -1, best_support = -1;
+    R best_score = R(-2);
+    for (int k = 0; k < nface; ++k) {
+        const int32_t q = phi[f[k]];
+        if (q == res) continue;
+        int support = 0;
+        for (int t = 0; t < ncube; ++t) support += (t != ccentre) && (cube[t] == q);
+        if (support > best_support || (support == best_support && score[k] > best_score)) {
+            best = k; best_support = support; best_score = score[k];
+        }
+    }
+    const int32_t qid = best >= 0 ? phi[f[best]] : res;
+    phi_comp[i] = qid;
+
+    // constitutive velocity v = M (dG - sigma(n) kappa) (src/gpk_physics.cpp:97-146)
+    const R gpv = g[i];
+    R gq = gpv, cq = c[i];
+    for (int k = 0; k < 6; ++k)
+        if (phi[f[k]] == qid) { gq = g[f[k]]; cq = c[f[k]]; break; }
+    g_comp[i] = gq;
+    const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
+    const R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
+    const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
+    const R mob = R(model.M0);
+    vraw[a] = mob * (dG - sig * kap);
+}
+
 template <class R, int DIM, bool kDirect>
 __global__ void __launch_bounds__(256)
 k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
            const int* __restrict__ band, const int32_t* __restrict__ phi,
            const R* __restrict__ w0, const R* __restrict__ w1, const R* __restrict__ w2,
-           const R* __restrict__ tab,
+           const R* __restrict__ tab, RowLutDesc lutd,
            const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
            R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
            R* __restrict__ okappa, R* __restrict__ vraw,
@@ -87,6 +200,7 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
     int z0, y0, x0;
     decode(d, i, z0, y0, x0);
     const int r = gp.rg, W = gp.W;
+    const LutRef<R> lut{lutd, tab};
     R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
     if (kDirect) {
         int t = 0;
@@ -119,13 +233,13 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
                 const int y = wrapi(y0 + dy, d.Ny);
                 const int rowbase = (z * d.Ny + y) * d.Nx;
                 const int jy = dy + r;
-                R R0 = 0, R1 = 0, R2 = 0;
+                uint64_t m = 0;
                 for (int ddx = -r; ddx <= r; ++ddx) {
                     const int x = wrapi(x0 + ddx, d.Nx);
-                    if (phi[rowbase + x] != res) continue;
-                    const int jx = ddx + r;
-                    R0 += w0[jx]; R1 += w1[jx]; R2 += w2[jx];
+                    m = m + m + (phi[rowbase + x] == res ? 1u : 0u);
                 }
+                R R0, R1, R2;
+                row_sum(lut, m, R0, R1, R2);
                 S00 = fmar(w0[jy], R0, S00); S01 = fmar(w0[jy], R1, S01);
                 S02 = fmar(w0[jy], R2, S02); S10 = fmar(w1[jy], R0, S10);
                 S11 = fmar(w1[jy], R1, S11); S20 = fmar(w2[jy], R0, S20);
@@ -144,74 +258,150 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
         }
     }
     (void)W;
-    const R dx = R(gp.dx_f);
-    const R inv_dx = R(1) / dx;
-    const R inv_dx2 = inv_dx * inv_dx;
-    const R klim = R(1) / (R(gp.sigma_cells) * dx);
-    Ix *= inv_dx; Iy *= inv_dx; Iz *= inv_dx;
-    Ixx *= inv_dx2; Iyy *= inv_dx2; Izz *= inv_dx2;
-    Ixy *= inv_dx2; Ixz *= inv_dx2; Iyz *= inv_dx2;
-    const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
-    R nxv = 0, nyv = 0, nzv = 0, kap = 0;
-    if (gsq * dx * dx > Eps<R>::grad) {
-        const R gr = dsqrt(gsq);
-        nxv = -Ix / gr; nyv = -Iy / gr; nzv = -Iz / gr;
-        const R trH = Ixx + Iyy + Izz;
-        const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
-                    + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
-        R kv = (gHg - gsq * trH) / (gsq * gr);
-        if (dfabs(kv * dx) < Eps<R>::kappa) kv = R(0);
-        kap = kv < -klim ? -klim : (klim < kv ? klim : kv);
-    }
-    onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
+    auto cube_at = [&](int t, int zz, int yy, int xx) {
+        (void)t;
+        return phi[lin(d, zz, yy, xx)];
+    };
+    finish_cell<R, DIM>(d, gp, model, i, a, z0, y0, x0, res, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz, Iyz,
+                        cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+}
 
-    // competitor: most-supported unlike face label, ties by outward-normal
-    // alignment, then first face (src/gpk_geometry.cpp:154-182)
-    int f[6];
-    faces(d, i, f);
-    const R score[6] = {-nxv, nxv, -nyv, nyv, -nzv, nzv};
-    const int nface = DIM == 3 ? 6 : 4;
-    int32_t cube[27];
-    {
-        int t = 0;
-        const int zb = DIM == 3 ? -1 : 0, ze = DIM == 3 ? 1 : 0;
-        for (int dz = zb; dz <= ze; ++dz) {
-            int zz = z0 + dz; if (zz < 0) zz += d.Nz; else if (zz >= d.Nz) zz -= d.Nz;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int yy = y0 + dy; if (yy < 0) yy += d.Ny; else if (yy >= d.Ny) yy -= d.Ny;
-                for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
-                    int xx = x0 + ddx; if (xx < 0) xx += d.Nx; else if (xx >= d.Nx) xx -= d.Nx;
-                    cube[t] = phi[lin(d, zz, yy, xx)];
+// ---------------------------------------------------------------------------
+// Separable geometry on LDS-staged tiles (FP32 / FP64 modes, labels < 2^16).
+// One workgroup per TZ x TY x TX tile: the tile's labels plus an RG halo are
+// staged as uint16 in LDS together with the row tables; the tile's interface
+// cells are compacted in LDS and each thread builds, per window row, the
+// W-bit "same label" mask (m = 2m + [lab == resident]) and reads the row sums
+// from the tables.  Same operation order as k_geometry<R,DIM,false>.
+// Tiles are assigned XCD-major: consecutive workgroup ids go round-robin over
+// the 8 XCDs, so tile t = (id % 8) * per_xcd + id / 8 keeps neighbouring
+// tiles (shared halos) in one XCD's L2.
+// ---------------------------------------------------------------------------
+template <int DIM> struct TileShape;
+template <> struct TileShape<3> { static constexpr int TZ = 8, TY = 8, TX = 32; };
+template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 32, TX = 64; };
+
+template <class R, int DIM, int RG>
+struct TiledLds {
+    static constexpr int TZ = TileShape<DIM>::TZ, TY = TileShape<DIM>::TY, TX = TileShape<DIM>::TX;
+    static constexpr int HZ = DIM == 3 ? RG : 0;
+    static constexpr int SZ = TZ + 2 * HZ, SY = TY + 2 * RG, SX = TX + 2 * RG;
+    static constexpr int W = 2 * RG + 1;
+    static constexpr int NCH = (W + 8) / 9;
+    static constexpr int C = (W + NCH - 1) / NCH;
+    static constexpr int lut_entries() {
+        int n = 0;
+        for (int k = 0; k < NCH; ++k) {
+            const int len = (W - k * C) < C ? (W - k * C) : C;
+            n += (1 << len) * 3;
+        }
+        return n;
+    }
+    static constexpr int NLUT = lut_entries();
+};
+
+template <class R, int DIM, int RG>
+__global__ void __launch_bounds__(256)
+k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, int ntz,
+                 const int32_t* __restrict__ phi, const uint8_t* __restrict__ mask,
+                 const int* __restrict__ slot, const R* __restrict__ w0,
+                 const R* __restrict__ w1, const R* __restrict__ w2,
+                 const R* __restrict__ lut_g,
+                 const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+                 R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
+                 R* __restrict__ okappa, R* __restrict__ vraw,
+                 int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
+    using S = TiledLds<R, DIM, RG>;
+    __shared__ uint16_t lab[S::SZ * S::SY * S::SX];
+    __shared__ R lut[S::NLUT];
+    __shared__ R wy[3][S::W];
+    __shared__ short list[S::TZ * S::TY * S::TX];
+    __shared__ int nlist;
+
+    const int ntiles = ntx * nty * ntz;
+    const int per_xcd = (ntiles + 7) / 8;
+    const int tile = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+    if (tile >= ntiles) return;
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int x0t = tx * S::TX, y0t = ty * S::TY, z0t = tz * S::TZ;
+
+    if (threadIdx.x == 0) nlist = 0;
+    for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e] = lut_g[e];
+    for (int e = threadIdx.x; e < S::W; e += blockDim.x) {
+        wy[0][e] = w0[e]; wy[1][e] = w1[e]; wy[2][e] = w2[e];
+    }
+    for (int e = threadIdx.x; e < S::SZ * S::SY * S::SX; e += blockDim.x) {
+        const int lx = e % S::SX, ly = (e / S::SX) % S::SY, lz = e / (S::SX * S::SY);
+        const int z = wrapi(z0t + lz - S::HZ, d.Nz);
+        const int y = wrapi(y0t + ly - RG, d.Ny);
+        const int x = wrapi(x0t + lx - RG, d.Nx);
+        lab[e] = uint16_t(phi[lin(d, z, y, x)]);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < S::TZ * S::TY * S::TX; e += blockDim.x) {
+        const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
+        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
+        if (x < d.Nx && y < d.Ny && z < d.Nz && mask[lin(d, z, y, x)]) {
+            const int pos = atomicAdd(&nlist, 1);
+            list[pos] = short(e);
+        }
+    }
+    __syncthreads();
+    const int n = nlist;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int e = list[q];
+        const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
+        const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
+        const int i = lin(d, z0, y0, x0);
+        const uint16_t p = lab[((lz + S::HZ) * S::SY + (ly + RG)) * S::SX + (lx + RG)];
+        R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
+        const int nzw = DIM == 3 ? S::W : 1;
+        for (int jz = 0; jz < nzw; ++jz) {
+            R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
+            const uint16_t* plane = lab + (lz + jz) * S::SY * S::SX;
+            for (int jy = 0; jy < S::W; ++jy) {
+                const uint16_t* row = plane + (ly + jy) * S::SX + lx;
+                uint32_t m = 0;
+#pragma unroll
+                for (int jx = 0; jx < S::W; ++jx) m = m + m + (row[jx] == p ? 1u : 0u);
+                R R0, R1, R2;
+                int base = 0;
+#pragma unroll
+                for (int k = 0; k < S::NCH; ++k) {
+                    const int start = k * S::C;
+                    const int len = (S::W - start) < S::C ? (S::W - start) : S::C;
+                    const int sh = S::W - start - len;
+                    const int ix = int((m >> sh) & ((1u << len) - 1u));
+                    const R* t = lut + base + ix * 3;
+                    if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }
+                    else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
+                    base += (1 << len) * 3;
                 }
+                const R gy0 = wy[0][jy], gy1 = wy[1][jy], gy2 = wy[2][jy];
+                S00 = fmar(gy0, R0, S00); S01 = fmar(gy0, R1, S01);
+                S02 = fmar(gy0, R2, S02); S10 = fmar(gy1, R0, S10);
+                S11 = fmar(gy1, R1, S11); S20 = fmar(gy2, R0, S20);
+            }
+            if (DIM == 3) {
+                const R gz0 = wy[0][jz], gz1 = wy[1][jz], gz2 = wy[2][jz];
+                Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
+                Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
+                Ixy = fmar(gz0, S11, Ixy);
+                Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
+                Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
+            } else {
+                Ix = S01; Iy = S10; Ixx = S02; Iyy = S20; Ixy = S11;
             }
         }
+        auto cube_at = [&](int t, int zz, int yy, int xx) -> int32_t {
+            (void)zz; (void)yy; (void)xx;
+            const int cz = DIM == 3 ? t / 9 - 1 : 0, cy = (t % 9) / 3 - 1, cx = t % 3 - 1;
+            return int32_t(lab[((lz + S::HZ + cz) * S::SY + (ly + RG + cy)) * S::SX + (lx + RG + cx)]);
+        };
+        finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, int32_t(p), Ix, Iy, Iz, Ixx, Iyy,
+                            Izz, Ixy, Ixz, Iyz, cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw,
+                            phi_comp, g_comp);
     }
-    const int ncube = DIM == 3 ? 27 : 9, ccentre = DIM == 3 ? 13 : 4;
-    int best = -1, best_support = -1;
-    R best_score = R(-2);
-    for (int k = 0; k < nface; ++k) {
-        const int32_t q = phi[f[k]];
-        if (q == res) continue;
-        int support = 0;
-        for (int t = 0; t < ncube; ++t) support += (t != ccentre) && (cube[t] == q);
-        if (support > best_support || (support == best_support && score[k] > best_score)) {
-            best = k; best_support = support; best_score = score[k];
-        }
-    }
-    const int32_t qid = best >= 0 ? phi[f[best]] : res;
-    phi_comp[i] = qid;
-
-    // constitutive velocity v = M (dG - sigma(n) kappa) (src/gpk_physics.cpp:97-146)
-    const R gpv = g[i];
-    R gq = gpv, cq = c[i];
-    for (int k = 0; k < 6; ++k)
-        if (phi[f[k]] == qid) { gq = g[f[k]]; cq = c[f[k]]; break; }
-    g_comp[i] = gq;
-    const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
-    const R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
-    const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
-    const R mob = R(model.M0);
-    vraw[a] = mob * (dG - sig * kap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1451,9 +1641,19 @@ __global__ void k_diffusion_part(Dims d, int dim3, const R* __restrict__ s, cons
 // ---------------------------------------------------------------------------
 // Label table maintenance and diagnostics
 // ---------------------------------------------------------------------------
+// per-label cell counts: each thread walks 64 consecutive cells and issues one
+// atomic per run of equal labels (neighbouring cells mostly share a label)
 __global__ void k_label_hist(int N, const int32_t* __restrict__ phi, int* __restrict__ count) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        atomicAdd(count + phi[i], 1);
+    const long long b = (long long)(blockIdx.x * blockDim.x + threadIdx.x) * 64;
+    if (b >= N) return;
+    const int e = b + 64 < N ? int(b + 64) : N;
+    int cur = phi[b], run = 0;
+    for (int i = int(b); i < e; ++i) {
+        const int v = phi[i];
+        if (v != cur) { atomicAdd(count + cur, run); cur = v; run = 0; }
+        ++run;
+    }
+    atomicAdd(count + cur, run);
 }
 
 __global__ void k_fill_i32(int n, int32_t* p, int32_t v) {

@@ -12,8 +12,9 @@
  *                     reference's operation order: bit-exact with the
  *                     reference step (pinned against the compiled reference
  *                     and its CSV fixtures by tests/).
- *   mode 1  FAST32    float fields, separable row-sum geometry with FMA;
- *                     everything else as STRICT32.
+ *   mode 1  FAST32    float fields, separable geometry: window rows as
+ *                     "same label" bit masks summed through chunk tables,
+ *                     FMA over y and z; everything else as STRICT32.
  *   mode 2  FP64      double fields and weights, separable geometry.
  *   mode 3  STRICT64  double fields, direct geometry order.
  * The GPU library implements modes 0-2 with identical operation order, so
@@ -157,6 +158,48 @@ void geometry_kernels(const Gauss<R>& g, std::vector<R>& w0, std::vector<R>& w1,
 }
 
 // ---------------------------------------------------------------------------
+// Row-sum tables of the separable modes (FP32, FP64).  The "same label" bits
+// of one window row are packed into m with dx = 0 at the most significant of
+// W bits (m = 2m + [phi == resident] in ascending dx).  The row is cut into
+// nch = ceil(W/9) chunks of C = ceil(W/nch) consecutive dx; chunk k's table
+// holds, for each bit pattern, the sum of its three weight sets over the set
+// bits in ascending dx order.  A row sum adds the chunk values in ascending
+// chunk order.
+// ---------------------------------------------------------------------------
+template <class R>
+struct RowLut {
+    int W = 0, C = 0, nch = 0;
+    int start[8], len[8], base[8];
+    std::vector<R> t;   // base[k] + idx*3 + a
+};
+
+template <class R>
+RowLut<R> make_row_lut(const std::vector<R>& w0, const std::vector<R>& w1, const std::vector<R>& w2) {
+    RowLut<R> L;
+    L.W = static_cast<int>(w0.size());
+    L.nch = (L.W + 8) / 9;
+    L.C = (L.W + L.nch - 1) / L.nch;
+    int off = 0;
+    for (int k = 0; k < L.nch; ++k) {
+        L.start[k] = k * L.C;
+        L.len[k] = std::min(L.C, L.W - L.start[k]);
+        L.base[k] = off;
+        off += (1 << L.len[k]) * 3;
+    }
+    L.t.assign(off, R(0));
+    const std::vector<R>* w[3] = {&w0, &w1, &w2};
+    for (int k = 0; k < L.nch; ++k)
+        for (int idx = 0; idx < (1 << L.len[k]); ++idx)
+            for (int a = 0; a < 3; ++a) {
+                R sum = R(0);
+                for (int j = 0; j < L.len[k]; ++j)
+                    if ((idx >> (L.len[k] - 1 - j)) & 1) sum += (*w[a])[L.start[k] + j];
+                L.t[L.base[k] + idx * 3 + a] = sum;
+            }
+    return L;
+}
+
+// ---------------------------------------------------------------------------
 // Constitutive laws (include/gpk_thermo.hpp:18-85, 229-325; src/gpk_thermo.cpp:9-30)
 // ---------------------------------------------------------------------------
 template <class R>
@@ -260,6 +303,7 @@ struct Grid {
     float vel_sigma, remap_sigma;
     std::vector<R> w0, w1, w2;
     int rg = 0;
+    RowLut<R> lut;
     std::vector<double> proj_w, remap_w;   // weight by integer r^2
     int proj_r = 1, remap_r = 1;
 
@@ -278,6 +322,7 @@ struct Grid {
         law.d = m;
         gauss = make_gauss<R>(sigma);
         geometry_kernels(gauss, w0, w1, w2, rg);
+        lut = make_row_lut(w0, w1, w2);
         // src/gpk_grid.cpp:24-27 (GaussKernel clamps sigma to >= 0.5)
         vel_sigma = std::max(Nz > 1 ? std::clamp(0.375f * sigma, 0.85f, 3.00f) : 0.85f, 0.5f);
         remap_sigma = std::max(Nz > 1 ? std::clamp(0.125f * sigma, 0.70f, 1.00f) : 0.70f, 0.5f);
@@ -395,12 +440,18 @@ struct Grid {
                     for (int dy = -r; dy <= r; ++dy) {
                         const int y = wrap(y0 + dy, Ny);
                         const int jy = dy + r;
-                        R R0 = 0, R1 = 0, R2 = 0;
+                        uint64_t m = 0;
                         for (int ddx = -r; ddx <= r; ++ddx) {
                             const int x = wrap(x0 + ddx, Nx);
-                            if (phi[idx(z, y, x)] != res) continue;
-                            const int jx = ddx + r;
-                            R0 += w0[jx]; R1 += w1[jx]; R2 += w2[jx];
+                            m = m + m + (phi[idx(z, y, x)] == res ? 1u : 0u);
+                        }
+                        R R0 = 0, R1 = 0, R2 = 0;
+                        for (int k = 0; k < lut.nch; ++k) {
+                            const int sh = lut.W - lut.start[k] - lut.len[k];
+                            const int ix = static_cast<int>((m >> sh) & ((1ull << lut.len[k]) - 1));
+                            const R* t = lut.t.data() + lut.base[k] + ix * 3;
+                            if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }
+                            else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
                         }
                         S00 = fma_r(w0[jy], R0, S00); S01 = fma_r(w0[jy], R1, S01);
                         S02 = fma_r(w0[jy], R2, S02); S10 = fma_r(w1[jy], R0, S10);

@@ -121,3 +121,20 @@ def test_anisotropic_growth_2d_labels_and_tolerance():
         assert np.array_equal(o.get("phi"), G.get("phi")), s
         vo, vg = o.get("vn").astype(np.float64), G.get("vn").astype(np.float64)
         assert np.all(np.abs(vo - vg) <= 1e-5 * np.maximum(np.abs(vo).max(), 1e-30)), s
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_fast_modes_generic_kernel_large_labels(mode):
+    # labels above 2^16 take the generic (non-tiled) separable kernel; the
+    # operation order is the same, so parity with the oracle is bit-exact
+    phi, g = O.voronoi(20, 24, 28, 30, 4)
+    phi = phi + 70000
+    o, G = pair((20, 24, 28), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 20, check_every=4)
+
+
+@pytest.mark.parametrize("sigma", [1.0, 2.6667])
+def test_fast_mode_other_radii_2d(sigma):
+    phi, g = O.voronoi(1, 128, 96, 40, 6)
+    o, G = pair((1, 128, 96), _abi.isotropic(1.0, 1.0), 0.0, sigma, "fp64", {"phi": phi, "g": g})
+    lockstep(o, G, 30, check_every=5)
