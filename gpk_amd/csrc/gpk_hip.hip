@@ -805,6 +805,14 @@ struct Engine final : EngineBase {
                 nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
     }
 
+    template <int DIM, class F>
+    void launch_small_tiled(F&& f) {
+        const int TZ = SmallTile<DIM>::TZ, TY = SmallTile<DIM>::TY, TX = SmallTile<DIM>::TX;
+        const int ntx = (Nx + TX - 1) / TX, nty = (Ny + TY - 1) / TY, ntz = (Nz + TZ - 1) / TZ;
+        const int ntiles = ntx * nty * ntz;
+        f(ntx, nty, ntz, 8 * ((ntiles + 7) / 8));
+    }
+
     // ---- band compaction: mask + sorted interface list (+ slot map) ----
     int band_compact(DevBuf& list, bool with_slot) {
         k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
@@ -840,7 +848,21 @@ struct Engine final : EngineBase {
             if (profiling) record_geo_end(nb);
         }
         k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
-        if (nb > 0) {
+        // the tiled projection packs (low, high, orientation) in 32 bits: labels < 2^15
+        if (nb > 0 && nlab <= 32768 && proj_r == 3) {
+            if (dim3) launch_small_tiled<3>([&](int ntx, int nty, int ntz, int grid) {
+                k_project_tiled<R, 3, 3><<<grid, 256, 0, stream>>>(
+                    dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                    slot.as<int>(), projw.as<double>(), vraw.as<R>(), vn.as<R>(),
+                    reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+            });
+            else launch_small_tiled<2>([&](int ntx, int nty, int ntz, int grid) {
+                k_project_tiled<R, 2, 3><<<grid, 256, 0, stream>>>(
+                    dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                    slot.as<int>(), projw.as<double>(), vraw.as<R>(), vn.as<R>(),
+                    reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+            });
+        } else if (nb > 0) {
             if (dim3)
                 k_project<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
                     dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(),
@@ -965,9 +987,21 @@ struct Engine final : EngineBase {
     // ---- balanced Gaussian remap sweep (src/gpk_transitions.cpp:38-178) ----
     int remap_sweep() {
         set_int(&ds()->exchanges, 0);
-        const int nb = band_compact(bandB, false);
+        const bool tiled = nlab <= 65536 && remap_r == 3;
+        const int nb = band_compact(bandB, tiled);
         k_copy_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), phi_comp.as<int32_t>());
-        if (nb > 0) {
+        if (nb > 0 && tiled) {
+            if (dim3) launch_small_tiled<3>([&](int ntx, int nty, int ntz, int grid) {
+                k_remap_score_tiled<R, 3, 3><<<grid, 256, 0, stream>>>(
+                    dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), slot.as<int>(),
+                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+            });
+            else launch_small_tiled<2>([&](int ntx, int nty, int ntz, int grid) {
+                k_remap_score_tiled<R, 2, 3><<<grid, 256, 0, stream>>>(
+                    dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), slot.as<int>(),
+                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+            });
+        } else if (nb > 0) {
             if (dim3)
                 k_remap_score<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
                     dims, remap_r, nb, bandB.as<int>(), phi.as<int32_t>(), remapw.as<double>(),

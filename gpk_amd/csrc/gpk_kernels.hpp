@@ -470,6 +470,212 @@ __global__ void k_vmax_masked(int nband, const int* __restrict__ band, const uin
     if (a < nband && mask[band[a]]) atomic_max_abs(vmax_bits, vn[a]);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-tiled projection and remap scoring (labels < 2^16).  A workgroup stages
+// a TZ x TY x TX tile plus an R-cell halo; each interface cell of the tile then
+// runs the reference's window sum in the reference order out of LDS.
+// ---------------------------------------------------------------------------
+template <int DIM> struct SmallTile;
+template <> struct SmallTile<3> { static constexpr int TZ = 4, TY = 8, TX = 32; };
+template <> struct SmallTile<2> { static constexpr int TZ = 1, TY = 32, TX = 32; };
+
+template <int DIM, int RH>
+struct HaloGeom {
+    static constexpr int TZ = SmallTile<DIM>::TZ, TY = SmallTile<DIM>::TY, TX = SmallTile<DIM>::TX;
+    static constexpr int HZ = DIM == 3 ? RH : 0;
+    static constexpr int SZ = TZ + 2 * HZ, SY = TY + 2 * RH, SX = TX + 2 * RH;
+    static constexpr int NS = SZ * SY * SX;
+    static constexpr int NT = TZ * TY * TX;
+};
+
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per_xcd = (ntiles + 7) / 8;
+    return (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+}
+
+// same-pair projection of v (src/gpk_physics.cpp:152-193) from LDS
+template <class R, int DIM, int RP>
+__global__ void __launch_bounds__(256)
+k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
+                const int32_t* __restrict__ phi, const int32_t* __restrict__ phi_comp,
+                const int* __restrict__ slot, const double* __restrict__ wtab_g,
+                const R* __restrict__ vraw, R* __restrict__ vout,
+                typename UBits<R>::T* __restrict__ vmax_bits) {
+    using H = HaloGeom<DIM, RP>;
+    __shared__ uint32_t key[H::NS];
+    __shared__ R vv[H::NS];
+    __shared__ double wt[3 * RP * RP + 1];
+    __shared__ short list[H::NT];
+    __shared__ int nlist;
+    const int ntiles = ntx * nty * ntz;
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
+    if (threadIdx.x == 0) nlist = 0;
+    for (int e = threadIdx.x; e < 3 * RP * RP + 1; e += blockDim.x) wt[e] = wtab_g[e];
+    // key = ((low << 16 | high) << 1 | [resident is the high label]) for
+    // interface cells whose two labels differ, all ones otherwise (labels
+    // below 2^15, so the packed pair fits 31 bits and never equals all ones)
+    for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
+        const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
+        const int j = lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RP, d.Ny),
+                          wrapi(x0t + lx - RP, d.Nx));
+        uint32_t k = 0xFFFFFFFFu;
+        R v = R(0);
+        if (mask[j]) {
+            const uint32_t p = uint32_t(phi[j]), q = uint32_t(phi_comp[j]);
+            if (p != q) {
+                const uint32_t lo = p < q ? p : q, hi = p < q ? q : p;
+                k = (((lo << 16) | hi) << 1) | (p == lo ? 0u : 1u);
+            }
+            v = vraw[slot[j]];
+        }
+        key[e] = k;
+        vv[e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < H::NT; e += blockDim.x) {
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
+        if (x < d.Nx && y < d.Ny && z < d.Nz && mask[lin(d, z, y, x)]) list[atomicAdd(&nlist, 1)] = short(e);
+    }
+    __syncthreads();
+    const int n = nlist;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int e = list[q];
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        const int c = ((lz + H::HZ) * H::SY + (ly + RP)) * H::SX + (lx + RP);
+        const uint32_t ki = key[c];
+        R out = vv[c];
+        if (ki != 0xFFFFFFFFu) {
+            const uint32_t pair = ki >> 1;
+            const R oi = (ki & 1u) ? R(-1) : R(1);
+            double num = 0.0, den = 0.0;
+            const int nzw = DIM == 3 ? 2 * RP + 1 : 1;
+            for (int jz = 0; jz < nzw; ++jz) {
+                const int dz = DIM == 3 ? jz - RP : 0;
+                for (int jy = 0; jy <= 2 * RP; ++jy) {
+                    const int dy = jy - RP;
+                    const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RP + dy)) * H::SX + lx;
+                    const double* wr = wt + dy * dy + dz * dz;
+#pragma unroll
+                    for (int jx = 0; jx <= 2 * RP; ++jx) {
+                        const int dxx = jx - RP;
+                        const uint32_t kj = key[rowb + jx];
+                        const bool hit = (kj >> 1) == pair;
+                        const double w = wr[dxx * dxx];
+                        const double wo = (kj & 1u) ? -w : w;
+                        // adding +0.0 leaves the sums unchanged (they never hold -0)
+                        num += hit ? wo * double(vv[rowb + jx]) : 0.0;
+                        den += hit ? w : 0.0;
+                    }
+                }
+            }
+            if (den > 0.0) out = oi * R(num / den);
+        }
+        const int i = lin(d, z0t + lz, y0t + ly, x0t + lx);
+        vout[slot[i]] = out;
+        atomic_max_abs(vmax_bits, out);
+    }
+}
+
+// resident / best-candidate Gaussian scores of the balanced remap
+// (src/gpk_transitions.cpp:53-91) from LDS
+template <class R, int DIM, int RS>
+__global__ void __launch_bounds__(256)
+k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
+                    const int32_t* __restrict__ phi, const int* __restrict__ slot,
+                    const double* __restrict__ wtab_g, int32_t* __restrict__ phi_comp,
+                    uint8_t* __restrict__ pflag, R* __restrict__ gain_s) {
+    using H = HaloGeom<DIM, RS>;
+    __shared__ uint16_t lab[H::NS];
+    __shared__ double wt[3 * RS * RS + 1];
+    __shared__ short list[H::NT];
+    __shared__ int nlist;
+    const int ntiles = ntx * nty * ntz;
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;
+    const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
+    const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
+    if (threadIdx.x == 0) nlist = 0;
+    for (int e = threadIdx.x; e < 3 * RS * RS + 1; e += blockDim.x) wt[e] = wtab_g[e];
+    for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
+        const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
+        lab[e] = uint16_t(phi[lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RS, d.Ny),
+                                  wrapi(x0t + lx - RS, d.Nx))]);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < H::NT; e += blockDim.x) {
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
+        if (x < d.Nx && y < d.Ny && z < d.Nz && mask[lin(d, z, y, x)]) list[atomicAdd(&nlist, 1)] = short(e);
+    }
+    __syncthreads();
+    const int n = nlist;
+    const int nface = DIM == 3 ? 6 : 4;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int e = list[q];
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        const int c = ((lz + H::HZ) * H::SY + (ly + RS)) * H::SX + (lx + RS);
+        const int32_t p = lab[c];
+        const int fo[6] = {-1, 1, -H::SX, H::SX, -H::SX * H::SY, H::SX * H::SY};
+        int32_t cand[6];
+        int nc = 0;
+        for (int k = 0; k < nface; ++k) {
+            const int32_t v = lab[c + fo[k]];
+            if (v == p) continue;
+            bool known = false;
+            for (int t = 0; t < nc; ++t) known |= cand[t] == v;
+            if (!known) cand[nc++] = v;
+        }
+        // acc[0] = resident score, acc[1 + t] = support of candidate t; the
+        // number of accumulators walked is the wavefront's largest nc + 1
+        int32_t L[7];
+        L[0] = p;
+#pragma unroll
+        for (int t = 0; t < 6; ++t) L[1 + t] = t < nc ? cand[t] : -1;
+        int ncw = nc;
+        for (int off = 32; off > 0; off >>= 1) {
+            const int o = __shfl_xor(ncw, off);
+            ncw = o > ncw ? o : ncw;
+        }
+        double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+        const int nzw = DIM == 3 ? 2 * RS + 1 : 1;
+        for (int jz = 0; jz < nzw; ++jz) {
+            const int dz = DIM == 3 ? jz - RS : 0;
+            for (int jy = 0; jy <= 2 * RS; ++jy) {
+                const int dy = jy - RS;
+                const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RS + dy)) * H::SX + lx;
+                const double* wr = wt + dy * dy + dz * dz;
+#pragma unroll
+                for (int jx = 0; jx <= 2 * RS; ++jx) {
+                    const int dxx = jx - RS;
+                    const int32_t v = lab[rowb + jx];
+                    const double w = wr[dxx * dxx];
+                    acc[0] += v == p ? w : 0.0;
+#pragma unroll
+                    for (int t = 0; t < 6; ++t)
+                        if (t < ncw) acc[1 + t] += v == L[1 + t] ? w : 0.0;
+                }
+            }
+        }
+        const double resident = acc[0];
+        double sup[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) sup[t] = acc[1 + t];
+        double best = -1.0;
+        int32_t bl = p;
+        for (int t = 0; t < nc; ++t) if (sup[t] > best) { best = sup[t]; bl = cand[t]; }
+        const int i = lin(d, z0t + lz, y0t + ly, x0t + lx);
+        phi_comp[i] = bl;
+        const R gain = R(best) - R(resident);
+        const int a = slot[i];
+        pflag[a] = (p != bl && gain > Eps<R>::gain) ? 1 : 0;
+        gain_s[a] = gain;
+    }
+}
+
 // scalar block shared by the step kernels (device memory)
 struct Scalars {
     unsigned long long vmax_bits;   // |v| max as bits (float bits in low word for FP32)
@@ -1017,20 +1223,28 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
                              int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
                              int all_accept) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= m) return;
-    const int cell = Ccell[k];
-    cand_id[cell] = -1;
-    if (all_accept || state[k] == ACCEPT) {
-        phi[cell] = Cnew[k];
-        g[cell] = Cg[k];
-        alpha[cell] = R(1);
-        atomicSub(count + Cold[k], 1);
-        atomicAdd(count + Cnew[k], 1);
-        atomicAdd(&s->accepted, 1);
-    } else {
-        const R an = Canew[k];
-        alpha[cell] = an < Eps<R>::flip ? Eps<R>::flip : an;
-        atomicAdd(&s->deferred, 1);
+    bool acc = false, def = false;
+    if (k < m) {
+        const int cell = Ccell[k];
+        cand_id[cell] = -1;
+        if (all_accept || state[k] == ACCEPT) {
+            phi[cell] = Cnew[k];
+            g[cell] = Cg[k];
+            alpha[cell] = R(1);
+            atomicSub(count + Cold[k], 1);
+            atomicAdd(count + Cnew[k], 1);
+            acc = true;
+        } else {
+            const R an = Canew[k];
+            alpha[cell] = an < Eps<R>::flip ? Eps<R>::flip : an;
+            def = true;
+        }
+    }
+    // one counter update per wavefront
+    const unsigned long long ba = __ballot(acc), bd = __ballot(def);
+    if ((threadIdx.x & 63) == 0) {
+        if (ba) atomicAdd(&s->accepted, __popcll(ba));
+        if (bd) atomicAdd(&s->deferred, __popcll(bd));
     }
 }
 

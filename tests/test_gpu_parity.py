@@ -138,3 +138,11 @@ def test_fast_mode_other_radii_2d(sigma):
     phi, g = O.voronoi(1, 128, 96, 40, 6)
     o, G = pair((1, 128, 96), _abi.isotropic(1.0, 1.0), 0.0, sigma, "fp64", {"phi": phi, "g": g})
     lockstep(o, G, 30, check_every=5)
+
+
+def test_projection_generic_path_labels_above_2_15():
+    # labels >= 2^15 take the untiled projection; same order, same bits
+    phi, g = O.voronoi(1, 96, 96, 30, 8)
+    phi = phi + 40000
+    o, G = pair((1, 96, 96), _abi.isotropic(1.0, 1.0), 0.0, 1.5, "strict32", {"phi": phi, "g": g})
+    lockstep(o, G, 30, check_every=5)
