@@ -178,7 +178,7 @@ std::vector<double> radial_table(float s, int r, bool single) {
 // Row-sum tables of the separable modes (same construction as the device
 // reads them, see RowLutDesc): nch = ceil(W/9) chunks of C = ceil(W/nch) dx,
 // entry base_k + idx*3 + a = sum of weight set a over the set bits of idx in
-// ascending dx (the most significant bit of idx is the chunk's first dx).
+// ascending dx (bit j of idx is the chunk's j-th dx).
 template <class R>
 RowLutDesc make_row_lut(const std::vector<R>& w0, const std::vector<R>& w1, const std::vector<R>& w2,
                         std::vector<R>& t) {
@@ -200,7 +200,7 @@ RowLutDesc make_row_lut(const std::vector<R>& w0, const std::vector<R>& w1, cons
             for (int a = 0; a < 3; ++a) {
                 R sum = R(0);
                 for (int j = 0; j < L.len[k]; ++j)
-                    if ((idx >> (L.len[k] - 1 - j)) & 1) sum += (*w[a])[L.start[k] + j];
+                    if ((idx >> j) & 1) sum += (*w[a])[L.start[k] + j];
                 t[L.base[k] + idx * 3 + a] = sum;
             }
     return L;
@@ -792,7 +792,7 @@ struct Engine final : EngineBase {
         }
     }
     void launch_geometry(int nb, const GeomParams& gp) {
-        if (!kDirect && nlab <= 65536 && (dim3 ? try_tiled<3>(gp) : try_tiled<2>(gp))) return;
+        if (!kDirect && (dim3 ? try_tiled<3>(gp) : try_tiled<2>(gp))) return;
         if (dim3)
             k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                 dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),

@@ -70,11 +70,12 @@ template <> struct Eps<double> {
 __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-// Row-sum tables of the separable modes: window-row "same label" bits m
-// (dx = 0 at the most significant of W bits) cut into nch chunks of at most C
-// consecutive dx; chunk k covers dx in [start_k, start_k + len_k) and its
-// table entry idx*3 + a holds the sum of weight set a over the set bits in
-// ascending dx.  The row sum adds the chunk values in ascending chunk order.
+// Row-sum tables of the separable modes: window-row "same label" bits m (bit
+// dx set when window position dx carries the resident label) cut into nch
+// chunks of at most C consecutive dx; chunk k covers dx in
+// [start_k, start_k + len_k) and its table entry idx*3 + a holds the sum of
+// weight set a over the set bits in ascending dx.  The row sum adds the chunk
+// values in ascending chunk order.
 struct RowLutDesc {
     int W, nch;
     int start[8], len[8], base[8];
@@ -84,8 +85,7 @@ template <class R>
 __device__ __forceinline__ void row_sum(const RowLutDesc& L, const R* __restrict__ t, uint64_t m,
                                         R& R0, R& R1, R& R2) {
     for (int k = 0; k < L.nch; ++k) {
-        const int sh = L.W - L.start[k] - L.len[k];
-        const int ix = int((m >> sh) & ((1ull << L.len[k]) - 1ull));
+        const int ix = int((m >> L.start[k]) & ((1ull << L.len[k]) - 1ull));
         const R* e = t + L.base[k] + ix * 3;
         if (k == 0) { R0 = e[0]; R1 = e[1]; R2 = e[2]; }
         else { R0 += e[0]; R1 += e[1]; R2 += e[2]; }
@@ -236,7 +236,7 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
                 uint64_t m = 0;
                 for (int ddx = -r; ddx <= r; ++ddx) {
                     const int x = wrapi(x0 + ddx, d.Nx);
-                    m = m + m + (phi[rowbase + x] == res ? 1u : 0u);
+                    if (phi[rowbase + x] == res) m |= 1ull << (ddx + r);
                 }
                 R R0, R1, R2;
                 row_sum(lut, m, R0, R1, R2);
@@ -267,25 +267,31 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
 }
 
 // ---------------------------------------------------------------------------
-// Separable geometry on LDS-staged tiles (FP32 / FP64 modes, labels < 2^16).
-// One workgroup per TZ x TY x TX tile: the tile's labels plus an RG halo are
-// staged as uint16 in LDS together with the row tables; the tile's interface
-// cells are compacted in LDS and each thread builds, per window row, the
-// W-bit "same label" mask (m = 2m + [lab == resident]) and reads the row sums
-// from the tables.  Same operation order as k_geometry<R,DIM,false>.
-// Tiles are assigned XCD-major: consecutive workgroup ids go round-robin over
-// the 8 XCDs, so tile t = (id % 8) * per_xcd + id / 8 keeps neighbouring
-// tiles (shared halos) in one XCD's L2.
+// Separable geometry from per-tile row masks (FP32 / FP64 modes).
+// One workgroup per TZ x TY x TX tile.  The tile's interface cells are listed
+// and their resident labels deduplicated in an LDS hash.  For a batch of up to
+// MAXRES residents, every staged row (tile + RG halo, SX <= 64 cells) is read
+// by one wavefront and turned into one 64-bit "same label" mask per resident
+// with a single compare + ballot (bit = staged column).  Each interface cell
+// then takes its window row mask as (row mask >> lx) and reads the row sums
+// from the chunk tables; y and z follow with FMA.  Same operation order as
+// k_geometry<R,DIM,false>.  Tiles are assigned XCD-major (tile t =
+// (id % 8) * per_xcd + id / 8) so neighbouring tiles share one XCD's L2.
 // ---------------------------------------------------------------------------
 template <int DIM> struct TileShape;
 template <> struct TileShape<3> { static constexpr int TZ = 8, TY = 8, TX = 32; };
-template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 32, TX = 64; };
+template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 64, TX = 32; };
+
+constexpr int GEO_MAXRES = 8;    // residents per mask batch
+constexpr int GEO_HCAP = 256;    // resident hash slots per tile
+constexpr int GEO_MAXB = GEO_HCAP / GEO_MAXRES;   // batches per tile
 
 template <class R, int DIM, int RG>
 struct TiledLds {
     static constexpr int TZ = TileShape<DIM>::TZ, TY = TileShape<DIM>::TY, TX = TileShape<DIM>::TX;
     static constexpr int HZ = DIM == 3 ? RG : 0;
     static constexpr int SZ = TZ + 2 * HZ, SY = TY + 2 * RG, SX = TX + 2 * RG;
+    static constexpr int NROWS = SZ * SY;
     static constexpr int W = 2 * RG + 1;
     static constexpr int NCH = (W + 8) / 9;
     static constexpr int C = (W + NCH - 1) / NCH;
@@ -298,7 +304,63 @@ struct TiledLds {
         return n;
     }
     static constexpr int NLUT = lut_entries();
+    static_assert(SX <= 64, "a staged row must fit one wavefront");
 };
+
+// stencil sums of one interface cell from its window row masks
+// (rowmask(jz, jy): bit jx set when window position jx carries the resident)
+template <class R, int DIM, int RG, class RowMask>
+__device__ __forceinline__ void tiled_cell(
+    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0,
+    int32_t p, const R* lut, const R (*wy)[2 * RG + 1], RowMask rowmask,
+    const int32_t* __restrict__ phi, const int* __restrict__ slot, const R* __restrict__ g,
+    const R* __restrict__ c, const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony,
+    R* __restrict__ onz, R* __restrict__ okappa, R* __restrict__ vraw,
+    int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
+    using S = TiledLds<R, DIM, RG>;
+    const int i = lin(d, z0, y0, x0);
+    R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
+    const int nzw = DIM == 3 ? S::W : 1;
+    for (int jz = 0; jz < nzw; ++jz) {
+        R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
+#pragma unroll 4
+        for (int jy = 0; jy < S::W; ++jy) {
+            const unsigned long long m = rowmask(jz, jy);
+            R R0, R1, R2;
+            int base = 0;
+#pragma unroll
+            for (int k = 0; k < S::NCH; ++k) {
+                const int start = k * S::C;
+                const int len = (S::W - start) < S::C ? (S::W - start) : S::C;
+                const int ix = int((m >> start) & ((1ull << len) - 1ull));
+                const R* t = lut + base + ix * 3;
+                if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }
+                else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
+                base += (1 << len) * 3;
+            }
+            const R gy0 = wy[0][jy], gy1 = wy[1][jy], gy2 = wy[2][jy];
+            S00 = fmar(gy0, R0, S00); S01 = fmar(gy0, R1, S01);
+            S02 = fmar(gy0, R2, S02); S10 = fmar(gy1, R0, S10);
+            S11 = fmar(gy1, R1, S11); S20 = fmar(gy2, R0, S20);
+        }
+        if (DIM == 3) {
+            const R gz0 = wy[0][jz], gz1 = wy[1][jz], gz2 = wy[2][jz];
+            Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
+            Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
+            Ixy = fmar(gz0, S11, Ixy);
+            Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
+            Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
+        } else {
+            Ix = S01; Iy = S10; Ixx = S02; Iyy = S20; Ixy = S11;
+        }
+    }
+    auto cube_at = [&](int t, int zz, int yy, int xx) -> int32_t {
+        (void)t;
+        return phi[lin(d, zz, yy, xx)];
+    };
+    finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, p, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz,
+                        Iyz, cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+}
 
 template <class R, int DIM, int RG>
 __global__ void __launch_bounds__(256)
@@ -312,11 +374,18 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  R* __restrict__ okappa, R* __restrict__ vraw,
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
-    __shared__ uint16_t lab[S::SZ * S::SY * S::SX];
+    constexpr int NT = S::TZ * S::TY * S::TX;
+    __shared__ unsigned long long rm[S::NROWS * GEO_MAXRES];
     __shared__ R lut[S::NLUT];
     __shared__ R wy[3][S::W];
-    __shared__ short list[S::TZ * S::TY * S::TX];
-    __shared__ int nlist;
+    __shared__ short list[NT];
+    __shared__ unsigned short cres[NT];
+    __shared__ short blist[NT];           // cells grouped by resident batch
+    __shared__ int hkey[GEO_HCAP];
+    __shared__ short hidx[GEO_HCAP];
+    __shared__ int resl[GEO_HCAP];
+    __shared__ int bcount[GEO_MAXB + 1], boff[GEO_MAXB + 1];
+    __shared__ int nlist, nres;
 
     const int ntiles = ntx * nty * ntz;
     const int per_xcd = (ntiles + 7) / 8;
@@ -325,25 +394,38 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0t = tx * S::TX, y0t = ty * S::TY, z0t = tz * S::TZ;
 
-    if (threadIdx.x == 0) nlist = 0;
+    if (threadIdx.x == 0) { nlist = 0; nres = 0; }
+    for (int e = threadIdx.x; e < GEO_HCAP; e += blockDim.x) hkey[e] = -1;
+    for (int e = threadIdx.x; e <= GEO_MAXB; e += blockDim.x) bcount[e] = 0;
     for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e] = lut_g[e];
     for (int e = threadIdx.x; e < S::W; e += blockDim.x) {
         wy[0][e] = w0[e]; wy[1][e] = w1[e]; wy[2][e] = w2[e];
     }
-    for (int e = threadIdx.x; e < S::SZ * S::SY * S::SX; e += blockDim.x) {
-        const int lx = e % S::SX, ly = (e / S::SX) % S::SY, lz = e / (S::SX * S::SY);
-        const int z = wrapi(z0t + lz - S::HZ, d.Nz);
-        const int y = wrapi(y0t + ly - RG, d.Ny);
-        const int x = wrapi(x0t + lx - RG, d.Nx);
-        lab[e] = uint16_t(phi[lin(d, z, y, x)]);
-    }
     __syncthreads();
-    for (int e = threadIdx.x; e < S::TZ * S::TY * S::TX; e += blockDim.x) {
+    // interface cells of the tile and their distinct resident labels
+    for (int e = threadIdx.x; e < NT; e += blockDim.x) {
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
         const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
-        if (x < d.Nx && y < d.Ny && z < d.Nz && mask[lin(d, z, y, x)]) {
-            const int pos = atomicAdd(&nlist, 1);
-            list[pos] = short(e);
+        if (x < d.Nx && y < d.Ny && z < d.Nz) {
+            const int i = lin(d, z, y, x);
+            if (mask[i]) {
+                list[atomicAdd(&nlist, 1)] = short(e);
+                const int p = phi[i];
+                int h = int((uint32_t(p) * 2654435761u) >> 23) & (GEO_HCAP - 1);
+                for (int probe = 0; probe < GEO_HCAP; ++probe) {
+                    const int prev = atomicCAS(&hkey[h], -1, p);
+                    if (prev == -1 || prev == p) break;
+                    h = (h + 1) & (GEO_HCAP - 1);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int h = threadIdx.x; h < GEO_HCAP; h += blockDim.x) {
+        if (hkey[h] != -1) {
+            const int k = atomicAdd(&nres, 1);
+            hidx[h] = short(k);
+            resl[k] = hkey[h];
         }
     }
     __syncthreads();
@@ -351,56 +433,79 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int e = list[q];
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
-        const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
-        const int i = lin(d, z0, y0, x0);
-        const uint16_t p = lab[((lz + S::HZ) * S::SY + (ly + RG)) * S::SX + (lx + RG)];
-        R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
-        const int nzw = DIM == 3 ? S::W : 1;
-        for (int jz = 0; jz < nzw; ++jz) {
-            R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
-            const uint16_t* plane = lab + (lz + jz) * S::SY * S::SX;
-            for (int jy = 0; jy < S::W; ++jy) {
-                const uint16_t* row = plane + (ly + jy) * S::SX + lx;
-                uint32_t m = 0;
-#pragma unroll
-                for (int jx = 0; jx < S::W; ++jx) m = m + m + (row[jx] == p ? 1u : 0u);
-                R R0, R1, R2;
-                int base = 0;
-#pragma unroll
-                for (int k = 0; k < S::NCH; ++k) {
-                    const int start = k * S::C;
-                    const int len = (S::W - start) < S::C ? (S::W - start) : S::C;
-                    const int sh = S::W - start - len;
-                    const int ix = int((m >> sh) & ((1u << len) - 1u));
-                    const R* t = lut + base + ix * 3;
-                    if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }
-                    else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
-                    base += (1 << len) * 3;
-                }
-                const R gy0 = wy[0][jy], gy1 = wy[1][jy], gy2 = wy[2][jy];
-                S00 = fmar(gy0, R0, S00); S01 = fmar(gy0, R1, S01);
-                S02 = fmar(gy0, R2, S02); S10 = fmar(gy1, R0, S10);
-                S11 = fmar(gy1, R1, S11); S20 = fmar(gy2, R0, S20);
-            }
-            if (DIM == 3) {
-                const R gz0 = wy[0][jz], gz1 = wy[1][jz], gz2 = wy[2][jz];
-                Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
-                Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
-                Ixy = fmar(gz0, S11, Ixy);
-                Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
-                Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
-            } else {
-                Ix = S01; Iy = S10; Ixx = S02; Iyy = S20; Ixy = S11;
+        const int p = phi[lin(d, z0t + lz, y0t + ly, x0t + lx)];
+        int h = int((uint32_t(p) * 2654435761u) >> 23) & (GEO_HCAP - 1);
+        unsigned short r = 0xFFFFu;     // 0xFFFF: the hash was full, no batch
+        for (int probe = 0; probe < GEO_HCAP; ++probe) {
+            if (hkey[h] == p) { r = (unsigned short)hidx[h]; break; }
+            if (hkey[h] == -1) break;
+            h = (h + 1) & (GEO_HCAP - 1);
+        }
+        cres[q] = r;
+        atomicAdd(&bcount[r == 0xFFFFu ? GEO_MAXB : r / GEO_MAXRES], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int o = 0;
+        for (int b = 0; b <= GEO_MAXB; ++b) { boff[b] = o; o += bcount[b]; bcount[b] = 0; }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int b = cres[q] == 0xFFFFu ? GEO_MAXB : cres[q] / GEO_MAXRES;
+        blist[boff[b] + atomicAdd(&bcount[b], 1)] = short(q);
+    }
+    const int nr = nres;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    for (int b0 = 0; b0 < nr; b0 += GEO_MAXRES) {
+        const int nbat = nr - b0 < GEO_MAXRES ? nr - b0 : GEO_MAXRES;
+        const int bb = b0 / GEO_MAXRES;
+        __syncthreads();
+        // row masks: wave per staged row, one ballot per resident of the batch
+        for (int r = wave; r < S::NROWS; r += nwave) {
+            const int lz = r / S::SY, ly = r % S::SY;
+            int v = -1;
+            if (lane < S::SX)
+                v = phi[lin(d, wrapi(z0t + lz - S::HZ, d.Nz), wrapi(y0t + ly - RG, d.Ny),
+                            wrapi(x0t + lane - RG, d.Nx))];
+            for (int t = 0; t < nbat; ++t) {
+                const unsigned long long m = __ballot(v == resl[b0 + t]);
+                if (lane == 0) rm[r * GEO_MAXRES + t] = m;
             }
         }
-        auto cube_at = [&](int t, int zz, int yy, int xx) -> int32_t {
-            (void)zz; (void)yy; (void)xx;
-            const int cz = DIM == 3 ? t / 9 - 1 : 0, cy = (t % 9) / 3 - 1, cx = t % 3 - 1;
-            return int32_t(lab[((lz + S::HZ + cz) * S::SY + (ly + RG + cy)) * S::SX + (lx + RG + cx)]);
-        };
-        finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, int32_t(p), Ix, Iy, Iz, Ixx, Iyy,
-                            Izz, Ixy, Ixz, Iyz, cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw,
-                            phi_comp, g_comp);
+        __syncthreads();
+        const int bend = boff[bb] + bcount[bb];
+        for (int u = boff[bb] + threadIdx.x; u < bend; u += blockDim.x) {
+            const int q = blist[u];
+            const int rs = int(cres[q]) - b0;
+            const int e = list[q];
+            const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
+            const unsigned long long* rows = rm + (lz * S::SY + ly) * GEO_MAXRES + rs;
+            tiled_cell<R, DIM, RG>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[cres[q]], lut, wy,
+                                   [&](int jz, int jy) {
+                                       return rows[(jz * S::SY + jy) * GEO_MAXRES] >> lx;
+                                   },
+                                   phi, slot, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+        }
+    }
+    // residents that did not fit the hash: window rows read from the grid
+    __syncthreads();
+    for (int u = boff[GEO_MAXB] + threadIdx.x; u < boff[GEO_MAXB] + bcount[GEO_MAXB]; u += blockDim.x) {
+        const int q = blist[u];
+        const int e = list[q];
+        const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
+        const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
+        const int32_t p = phi[lin(d, z0, y0, x0)];
+        tiled_cell<R, DIM, RG>(d, gp, model, x0, y0, z0, p, lut, wy,
+                               [&](int jz, int jy) {
+                                   const int z = DIM == 3 ? wrapi(z0 + jz - RG, d.Nz) : z0;
+                                   const int y = wrapi(y0 + jy - RG, d.Ny);
+                                   unsigned long long m = 0;
+                                   for (int jx = 0; jx < S::W; ++jx)
+                                       if (phi[lin(d, z, y, wrapi(x0 + jx - RG, d.Nx))] == p)
+                                           m |= 1ull << jx;
+                                   return m;
+                               },
+                               phi, slot, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
     }
 }
 

@@ -159,8 +159,8 @@ void geometry_kernels(const Gauss<R>& g, std::vector<R>& w0, std::vector<R>& w1,
 
 // ---------------------------------------------------------------------------
 // Row-sum tables of the separable modes (FP32, FP64).  The "same label" bits
-// of one window row are packed into m with dx = 0 at the most significant of
-// W bits (m = 2m + [phi == resident] in ascending dx).  The row is cut into
+// of one window row are packed into m with bit dx set when window position dx
+// (0 = leftmost) carries the resident label.  The row is cut into
 // nch = ceil(W/9) chunks of C = ceil(W/nch) consecutive dx; chunk k's table
 // holds, for each bit pattern, the sum of its three weight sets over the set
 // bits in ascending dx order.  A row sum adds the chunk values in ascending
@@ -193,7 +193,7 @@ RowLut<R> make_row_lut(const std::vector<R>& w0, const std::vector<R>& w1, const
             for (int a = 0; a < 3; ++a) {
                 R sum = R(0);
                 for (int j = 0; j < L.len[k]; ++j)
-                    if ((idx >> (L.len[k] - 1 - j)) & 1) sum += (*w[a])[L.start[k] + j];
+                    if ((idx >> j) & 1) sum += (*w[a])[L.start[k] + j];
                 L.t[L.base[k] + idx * 3 + a] = sum;
             }
     return L;
@@ -443,11 +443,11 @@ struct Grid {
                         uint64_t m = 0;
                         for (int ddx = -r; ddx <= r; ++ddx) {
                             const int x = wrap(x0 + ddx, Nx);
-                            m = m + m + (phi[idx(z, y, x)] == res ? 1u : 0u);
+                            if (phi[idx(z, y, x)] == res) m |= 1ull << (ddx + r);
                         }
                         R R0 = 0, R1 = 0, R2 = 0;
                         for (int k = 0; k < lut.nch; ++k) {
-                            const int sh = lut.W - lut.start[k] - lut.len[k];
+                            const int sh = lut.start[k];
                             const int ix = static_cast<int>((m >> sh) & ((1ull << lut.len[k]) - 1));
                             const R* t = lut.t.data() + lut.base[k] + ix * 3;
                             if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }

@@ -146,3 +146,14 @@ def test_projection_generic_path_labels_above_2_15():
     phi = phi + 40000
     o, G = pair((1, 96, 96), _abi.isotropic(1.0, 1.0), 0.0, 1.5, "strict32", {"phi": phi, "g": g})
     lockstep(o, G, 30, check_every=5)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_geometry_tile_hash_overflow_path(mode):
+    # random labels: every cell is an interface cell and a geometry tile holds
+    # far more distinct residents than its hash; those cells take the in-kernel
+    # path that reads window rows from the grid (same arithmetic)
+    rng = np.random.default_rng(3)
+    phi = rng.integers(0, 3000, size=(1, 96, 96)).astype(np.int32).reshape(96, 96)
+    o, G = pair((1, 96, 96), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi})
+    lockstep(o, G, 2)
