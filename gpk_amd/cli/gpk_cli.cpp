@@ -1,0 +1,360 @@
+// gpk_cli.cpp — `gpk <deck.ini>` driver on the GPU facade.
+//
+// Runs the two in-scope models of the reference CLI (demos/gpk.cpp):
+// [run] model = grain_growth (run_grain_growth, 195-282) and
+// [run] model = solidification (run_solidification, 288-391), with the same
+// deck keys, initial conditions, progress tables and snapshot files
+// ([output] format = csv | vtk | ppm | all, dir, prefix, write_initial).
+// Optional keys: [gpu] precision = strict32 | fp32 | fp64, [gpu] device.
+// Other models exit with status 1, as the reference does for unknown models.
+#include "gpk_core.hpp"
+#include "gpk_thermo.hpp"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+using namespace gpk;
+
+namespace {
+
+// INI decks (utils/input_parser.hpp:52-147): "[section]", "key = value",
+// '#' starts a comment, typed getters, a missing key without a default throws.
+class Deck {
+  public:
+    explicit Deck(const std::string& path) {
+        std::ifstream f(path);
+        if (!f) throw std::runtime_error("InputParser: cannot open '" + path + "'");
+        std::string section, line;
+        while (std::getline(f, line)) {
+            const auto hash = line.find('#');
+            if (hash != std::string::npos) line.erase(hash);
+            line = trim(line);
+            if (line.empty()) continue;
+            if (line.front() == '[' && line.back() == ']') {
+                section = trim(line.substr(1, line.size() - 2));
+                continue;
+            }
+            const auto eq = line.find('=');
+            if (eq == std::string::npos) continue;
+            kv_[section + "/" + trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+        }
+    }
+    bool has(const std::string& s, const std::string& k) const { return kv_.count(s + "/" + k) > 0; }
+    std::string str(const std::string& s, const std::string& k, const std::string& def = "") const {
+        auto it = kv_.find(s + "/" + k);
+        if (it == kv_.end()) {
+            if (def.empty()) throw missing(s, k);
+            return def;
+        }
+        return it->second;
+    }
+    int i(const std::string& s, const std::string& k, int def = -999999) const {
+        auto it = kv_.find(s + "/" + k);
+        if (it == kv_.end()) {
+            if (def == -999999) throw missing(s, k);
+            return def;
+        }
+        return std::stoi(it->second);
+    }
+    double d(const std::string& s, const std::string& k, double def = 1e300) const {
+        auto it = kv_.find(s + "/" + k);
+        if (it == kv_.end()) {
+            if (def > 1e299) throw missing(s, k);
+            return def;
+        }
+        return std::stod(it->second);
+    }
+    float f(const std::string& s, const std::string& k, float def = 1e30f) const {
+        return static_cast<float>(d(s, k, def < 1e29f ? double(def) : 1e300));
+    }
+    bool b(const std::string& s, const std::string& k, bool def = false) const {
+        auto it = kv_.find(s + "/" + k);
+        if (it == kv_.end()) return def;
+        std::string v = it->second;
+        std::transform(v.begin(), v.end(), v.begin(), ::tolower);
+        return v == "true" || v == "1" || v == "yes";
+    }
+
+  private:
+    std::unordered_map<std::string, std::string> kv_;
+    static std::string trim(const std::string& s) {
+        const auto b = s.find_first_not_of(" \t\r\n");
+        if (b == std::string::npos) return "";
+        const auto e = s.find_last_not_of(" \t\r\n");
+        return s.substr(b, e - b + 1);
+    }
+    static std::runtime_error missing(const std::string& s, const std::string& k) {
+        return std::runtime_error("InputParser: missing required key [" + s + "] " + k);
+    }
+};
+
+void mkdir_p(const std::string& path) {
+    std::string cur;
+    for (char ch : path + "/") {
+        if ((ch == '/' || ch == '\\') && !cur.empty()) ::mkdir(cur.c_str(), 0755);
+        cur += ch;
+    }
+}
+
+// Snapshot files.  CSV (utils/csv_writer.hpp:20-56) and legacy VTK
+// (utils/vtk_writer.hpp:18-67) carry the reference's layout and number
+// formats; PPM writes a mid-plane label image (own colouring).
+class Writer {
+  public:
+    explicit Writer(const Deck& cfg)
+        : format_(cfg.str("output", "format", "vtk")), dir_(cfg.str("output", "dir", "output")),
+          prefix_(cfg.str("output", "prefix", "snap")), compact_(cfg.b("output", "compact", false)) {
+        mkdir_p(dir_);
+    }
+    void write(GPKGrid& grid, int snap, double t) const {
+        grid.sync_host();
+        char tag[64];
+        std::snprintf(tag, sizeof(tag), "%04d", snap);
+        const std::string base = dir_ + "/" + prefix_ + "_" + tag;
+        if (format_ == "csv" || format_ == "all") csv(grid, base + ".csv", t, snap);
+        if (format_ == "vtk" || format_ == "all") vtk(grid, base + ".vtk", t, snap);
+        if (format_ == "ppm" || format_ == "all") ppm(grid, base + "_grains.ppm");
+    }
+
+  private:
+    std::string format_, dir_, prefix_;
+    bool compact_;
+
+    void csv(const GPKGrid& grid, const std::string& path, double t, int snap) const {
+        std::ofstream f(path);
+        if (!f) throw std::runtime_error("CSV: cannot open " + path);
+        const float dx = grid.dx;
+        f << "# GPK Microstructure CSV\n";
+        f << "# snap=" << snap << "  time=" << t << "\n";
+        f << "# Nz=" << grid.Nz << "  Ny=" << grid.Ny << "  Nx=" << grid.Nx << "\n";
+        f << "# dx=" << dx << " [m]\n";
+        if (compact_) f << "# compact=true (only interface + changed cells written)\n";
+        f << "# cols: x[m] y[m] z[m] grain_id alpha composition temperature orientation\n";
+        char row[256];
+        for (int z = 0; z < grid.Nz; ++z)
+            for (int y = 0; y < grid.Ny; ++y)
+                for (int x = 0; x < grid.Nx; ++x) {
+                    const int i = grid.idx(z, y, x);
+                    if (compact_ && !grid.mask[i]) continue;
+                    std::snprintf(row, sizeof(row), "%.6e %.6e %.6e %d %.6f %.6f %.2f %.6f\n", x * dx,
+                                  y * dx, z * dx, grid.phi[i], grid.alpha[i], grid.c[i], grid.T[i],
+                                  grid.g[i]);
+                    f << row;
+                }
+    }
+
+    void vtk(const GPKGrid& grid, const std::string& path, double t, int snap) const {
+        std::ofstream f(path);
+        if (!f) throw std::runtime_error("VTK: cannot open " + path);
+        const int N = grid.N();
+        f << "# vtk DataFile Version 3.0\n";
+        f << "GPK Microstructure snap=" << snap << " time=" << t << "\n";
+        f << "ASCII\nDATASET STRUCTURED_POINTS\n";
+        f << "DIMENSIONS " << grid.Nx << " " << grid.Ny << " " << grid.Nz << "\n";
+        f << "ORIGIN 0 0 0\n";
+        f << "SPACING " << grid.dx << " " << grid.dx << " " << grid.dx << "\n";
+        f << "POINT_DATA " << N << "\n";
+        f << "SCALARS grain_id int 1\nLOOKUP_TABLE default\n";
+        for (int i = 0; i < N; ++i) f << grid.phi[i] << ((i % 9 == 8) ? '\n' : ' ');
+        f << '\n';
+        const std::vector<float>* fields[4] = {&grid.alpha, &grid.c, &grid.T, &grid.g};
+        const char* names[4] = {"alpha", "composition", "temperature", "orientation"};
+        char buf[32];
+        for (int k = 0; k < 4; ++k) {
+            f << "SCALARS " << names[k] << " float 1\nLOOKUP_TABLE default\n";
+            for (int i = 0; i < N; ++i) {
+                std::snprintf(buf, sizeof(buf), "%.6g", (*fields[k])[i]);
+                f << buf << ((i % 6 == 5) ? '\n' : ' ');
+            }
+            f << '\n';
+        }
+    }
+
+    void ppm(const GPKGrid& grid, const std::string& path) const {
+        const int z0 = grid.Nz / 2, W = grid.Nx, H = grid.Ny;
+        std::vector<unsigned char> rgb(size_t(W) * H * 3);
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const int i = grid.idx(z0, y, x);
+                const uint32_t hsh = uint32_t(grid.phi[i]) * 2654435761u;
+                const float a = grid.mask[i] ? grid.alpha[i] : 1.f;
+                unsigned char* p = &rgb[(size_t(y) * W + x) * 3];
+                p[0] = (unsigned char)(a * float(64 + (hsh >> 8 & 191)));
+                p[1] = (unsigned char)(a * float(64 + (hsh >> 16 & 191)));
+                p[2] = (unsigned char)(a * float(64 + (hsh >> 24 & 191)));
+            }
+        std::ofstream f(path, std::ios::binary);
+        f << "P6\n" << W << " " << H << "\n255\n";
+        f.write(reinterpret_cast<const char*>(rgb.data()), std::streamsize(rgb.size()));
+    }
+};
+
+int precision_of(const Deck& cfg) {
+    const std::string p = cfg.str("gpu", "precision", "strict32");
+    if (p == "strict32") return GPK_PREC_STRICT32;
+    if (p == "fp32") return GPK_PREC_FP32;
+    if (p == "fp64") return GPK_PREC_FP64;
+    throw std::runtime_error("[gpu] precision must be strict32, fp32 or fp64");
+}
+
+// demos/gpk.cpp:195-282
+int run_grain_growth(const Deck& cfg) {
+    const int Nz = cfg.i("grid", "Nz"), Ny = cfg.i("grid", "Ny"), Nx = cfg.i("grid", "Nx");
+    const float dx = cfg.f("grid", "dx");
+    const double t_end = cfg.d("time", "t_end");
+    const int n_snaps = cfg.i("time", "n_snaps");
+    const float M0 = cfg.f("thermo", "M0"), sigma0 = cfg.f("thermo", "sigma0");
+    const float eps4 = cfg.f("thermo", "eps4", 0.f);
+    const int n_grains = cfg.i("grain_growth", "n_grains");
+    const int seed = cfg.i("grain_growth", "seed", 42);
+
+    std::printf("=== GPK Grain Growth (MI355X) ===\n");
+    std::printf("Grid %d×%d×%d, dx=%.1f µm, %d grains\n", Nz, Ny, Nx, dx * 1e6f, n_grains);
+    std::shared_ptr<ThermoModel> thermo;
+    if (std::fabs(eps4) > 1e-12f) thermo = std::make_shared<AnisotropicGrowthThermo>(M0, sigma0, eps4);
+    else thermo = std::make_shared<IsotropicGrowthThermo>(M0, sigma0);
+    GPKGrid grid(Nz, Ny, Nx, dx, thermo, 0.f, cfg.f("geometry", "sigma_cells", 2.f), precision_of(cfg),
+                 cfg.i("gpu", "device", 0));
+    if (cfg.has("grain_growth", "restart_labels")) {
+        auto read_exact = [](const std::string& path, char* dst, std::streamsize bytes) {
+            std::ifstream s(path, std::ios::binary);
+            if (!s) throw std::runtime_error("Cannot open restart field '" + path + "'");
+            s.read(dst, bytes);
+            if (s.gcount() != bytes) throw std::runtime_error("Restart field has the wrong size: '" + path + "'");
+            char extra = 0;
+            if (s.read(&extra, 1)) throw std::runtime_error("Restart field contains trailing data: '" + path + "'");
+        };
+        read_exact(cfg.str("grain_growth", "restart_labels"), reinterpret_cast<char*>(grid.phi.data()),
+                   std::streamsize(grid.phi.size() * sizeof(grid.phi[0])));
+        const std::string gpath = cfg.str("grain_growth", "restart_orientations", "none");
+        if (gpath != "none")
+            read_exact(gpath, reinterpret_cast<char*>(grid.g.data()),
+                       std::streamsize(grid.g.size() * sizeof(grid.g[0])));
+        std::fill(grid.alpha.begin(), grid.alpha.end(), 1.f);
+        grid.time = cfg.d("grain_growth", "restart_time", 0.0);
+        grid.mark_dirty();
+        std::printf("Restarted categorical state at t=%.4f s\n", grid.time);
+    } else {
+        grid.voronoi_init(n_grains, uint32_t(seed));
+    }
+    Writer writer(cfg);
+    const bool write_initial = cfg.b("output", "write_initial", false);
+    const int off = write_initial ? 1 : 0;
+    if (write_initial) writer.write(grid, 0, grid.time);
+    std::printf("%-12s %-18s %-10s\n", "Time [s]", "R_char [µm]", "Grains");
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    for (int snap = 0; snap < n_snaps; ++snap) {
+        const double t_snap = t_end * (snap + 1.0) / n_snaps;
+        grid.run(t_snap, -1.f,
+                 [](const GPKGrid& st, int steps) {
+                     std::printf("  progress: step %d, t=%.6g s, grains=%d\n", steps, st.time, st.n_grains());
+                     std::fflush(stdout);
+                 },
+                 50);
+        if (grid.n_grains() <= 1) break;
+        std::printf("%-12.4f %-18.4f %-10d\n", grid.time, grid.mean_grain_radius() * 1e6, grid.n_grains());
+        writer.write(grid, snap + off, grid.time);
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    std::printf("\nWall time: %.2f s\n", wall);
+    return 0;
+}
+
+// demos/gpk.cpp:288-391
+int run_solidification(const Deck& cfg) {
+    const int Nz = cfg.i("grid", "Nz"), Ny = cfg.i("grid", "Ny"), Nx = cfg.i("grid", "Nx");
+    const float dx = cfg.f("grid", "dx");
+    const double t_end = cfg.d("time", "t_end");
+    const int nsnap = cfg.i("time", "n_snaps");
+    const float M0 = cfg.f("thermo", "M0"), sig0 = cfg.f("thermo", "sigma0");
+    const float eps4 = cfg.f("thermo", "eps4", 0.04f);
+    const float D = cfg.f("thermo", "D"), m_liq = cfg.f("thermo", "m_liq");
+    const float k_part = cfg.f("thermo", "k_part"), T_melt = cfg.f("thermo", "T_melt");
+    const float L = cfg.f("thermo", "L");
+    const float c0 = cfg.f("initial", "c0"), dT = cfg.f("initial", "undercooling");
+    const int seed_r = cfg.i("initial", "seed_radius", 4);
+    if (cfg.b("thermo", "elastic_enabled", false))
+        throw std::runtime_error("elastic coupling has no device form in this build");
+
+    std::printf("=== GPK Solidification (MI355X) ===\n");
+    std::printf("Grid %d×%d×%d, dx=%.2f µm\n", Nz, Ny, Nx, dx * 1e6f);
+    auto thermo = std::make_shared<SolidificationThermo>(M0, sig0, eps4, m_liq, k_part, T_melt, L);
+    thermo->stochastic = cfg.b("thermo", "stochastic", false);
+    thermo->noise_scale = cfg.f("thermo", "noise_scale", 1.0f);
+    thermo->solute_rejection = cfg.b("thermo", "solute_rejection", false);
+    thermo->k_partition = k_part;
+    if (thermo->solute_rejection) std::printf("Solute partitioning enabled, k = %.3f\n", double(k_part));
+    GPKGrid grid(Nz, Ny, Nx, dx, thermo, D, cfg.f("geometry", "sigma_cells", 2.f), precision_of(cfg),
+                 cfg.i("gpu", "device", 0));
+    const float T_liq = T_melt + m_liq * c0;
+    std::fill(grid.c.begin(), grid.c.end(), c0);
+    std::fill(grid.T.begin(), grid.T.end(), T_liq - dT);
+    const float R_star = sig0 * T_melt / (L * dT);
+    std::printf("Gibbs-Thomson R* = %.3f µm\n", R_star * 1e6f);
+    std::printf("Seed radius      = %.1f µm (must be > R*)\n\n", seed_r * dx * 1e6f);
+    const int cz = Nz / 2, cy = Ny / 2, cx = Nx / 2;
+    for (int dz = -seed_r; dz <= seed_r; ++dz)
+        for (int dy = -seed_r; dy <= seed_r; ++dy)
+            for (int dxs = -seed_r; dxs <= seed_r; ++dxs) {
+                if (dz * dz + dy * dy + dxs * dxs > seed_r * seed_r) continue;
+                const int zz = cz + dz, yy = cy + dy, xx = cx + dxs;
+                if (zz < 0 || zz >= Nz || yy < 0 || yy >= Ny || xx < 0 || xx >= Nx) continue;
+                const int i = grid.idx(zz, yy, xx);
+                grid.phi[i] = 1;
+                grid.c[i] = c0 * k_part;
+            }
+    grid.mark_dirty();
+    Writer writer(cfg);
+    const bool write_initial = cfg.b("output", "write_initial", false);
+    const int off = write_initial ? 1 : 0;
+    if (write_initial) writer.write(grid, 0, grid.time);
+    const double snap_dt = t_end / nsnap;
+    std::printf("%-10s %-14s %-14s %-10s\n", "Time[ms]", "Solid frac", "Solute max", "Steps");
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    int total = 0;
+    for (int snap = 0; snap < nsnap; ++snap) {
+        const int steps = grid.run(snap_dt * (snap + 1), -1.f);
+        total += steps;
+        grid.sync_host();
+        const float sf = grid.solid_fraction();
+        const float cmax = *std::max_element(grid.c.begin(), grid.c.end());
+        std::printf("%-10.3f %-14.4f %-14.4f %-10d\n", grid.time * 1e3, sf, cmax, steps);
+        writer.write(grid, snap + off, grid.time);
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    std::printf("\nWall time: %.2f s  (%d steps, %.2f ms/step)\n", wall, total,
+                total ? 1e3 * wall / total : 0.0);
+    return 0;
+}
+
+} // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::fprintf(stderr, "Usage: %s <input.ini>\n", argv[0]);
+        return 1;
+    }
+    try {
+        Deck cfg(argv[1]);
+        const std::string model = cfg.str("run", "model");
+        if (model == "grain_growth") return run_grain_growth(cfg);
+        if (model == "solidification") return run_solidification(cfg);
+        std::fprintf(stderr, "Unknown or unsupported model '%s' (this build runs grain_growth and "
+                             "solidification)\n", model.c_str());
+        return 1;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "Error: %s\n", e.what());
+        return 1;
+    }
+}

@@ -1,0 +1,108 @@
+#pragma once
+/*
+ * gpk_thermo.hpp — the constitutive models of the GPU path with the
+ * reference's class names, constructors and parameters
+ * (include/gpk_thermo.hpp in the GPK source tree: 18-85, 229-325).  Each
+ * states itself to the HIP library through device_descriptor(); couplings the
+ * device path does not evaluate (elastic, dislocation, magnetic, phase
+ * selection, Arrhenius mobility) make device_descriptor() return false.
+ */
+#include "gpk_core.hpp"
+
+namespace gpk {
+
+// Isotropic grain growth, dG = 0, M = M0 (include/gpk_thermo.hpp:18-27)
+struct IsotropicGrowthThermo : ThermoModel {
+    explicit IsotropicGrowthThermo(float M0_val, float sigma0_val = 0.5f) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = 0.f;
+        grain_network = true;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_ISOTROPIC);
+        return true;
+    }
+};
+
+// Anisotropic grain growth, cubic / 4-fold sigma (include/gpk_thermo.hpp:33-45)
+struct AnisotropicGrowthThermo : ThermoModel {
+    float eps6 = 0.f;   // 2-D six-fold term (extension; 0 = the reference law)
+    AnisotropicGrowthThermo(float M0_val, float sigma0_val, float eps4_val) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = eps4_val;
+        grain_network = true;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_ANISO_GROWTH);
+        d->eps6 = eps6;
+        return true;
+    }
+};
+
+// Binary-alloy solidification, 0 = liquid, 1 = solid (include/gpk_thermo.hpp:52-85)
+struct SolidificationThermo : ThermoModel {
+    float m_liq, k_part, T_melt, L;
+    SolidificationThermo(float M0_val = 5e-8f, float sigma0_v = 0.2f, float eps4_v = 0.04f,
+                         float m_liq_v = -500.f, float k_part_v = 0.3f, float T_melt_v = 933.f,
+                         float L_v = 1e8f)
+        : m_liq(m_liq_v), k_part(k_part_v), T_melt(T_melt_v), L(L_v) {
+        M0 = M0_val;
+        sigma0 = sigma0_v;
+        eps4 = eps4_v;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_SOLIDIFICATION);
+        d->m_liq = m_liq;
+        d->k_part = k_part;
+        d->T_melt = T_melt;
+        d->L = L;
+        return true;
+    }
+};
+
+// Tensor interfacial energy with an optional volume bias (include/gpk_thermo.hpp:229-263)
+struct TensorAnisotropyThermo : ThermoModel {
+    float bxx, byy, bzz;
+    mutable float dG_bias = 0.f;
+    TensorAnisotropyThermo(float M0_val, float sigma0_val, float bxx_val = 1.f, float byy_val = 1.f,
+                           float bzz_val = 1.f)
+        : bxx(bxx_val), byy(byy_val), bzz(bzz_val) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = 0.f;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_TENSOR_ANISO);
+        d->bxx = bxx; d->byy = byy; d->bzz = bzz;
+        d->dG_bias = dG_bias;
+        return true;
+    }
+};
+
+// Tensor sigma + volume-conserving composition coupling (include/gpk_thermo.hpp:284-325)
+struct AnisoOstwaldThermo : ThermoModel {
+    float bxx, byy, bzz;
+    float c_sat, Omega;
+    AnisoOstwaldThermo(float M0_val, float sigma0_val, float bxx_val, float byy_val, float bzz_val,
+                       float c_sat_val, float Omega_val)
+        : bxx(bxx_val), byy(byy_val), bzz(bzz_val), c_sat(c_sat_val), Omega(Omega_val) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = 0.f;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_ANISO_OSTWALD);
+        d->bxx = bxx; d->byy = byy; d->bzz = bzz;
+        d->c_sat = c_sat; d->Omega = Omega;
+        return true;
+    }
+};
+
+} // namespace gpk
