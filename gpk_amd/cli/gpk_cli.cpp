@@ -109,7 +109,7 @@ void mkdir_p(const std::string& path) {
 
 // Snapshot files.  CSV (utils/csv_writer.hpp:20-56) and legacy VTK
 // (utils/vtk_writer.hpp:18-67) carry the reference's layout and number
-// formats; PPM writes a mid-plane label image (own colouring).
+// formats; PPM images follow utils/ppm_writer.hpp's colouring.
 class Writer {
   public:
     explicit Writer(const Deck& cfg)
@@ -124,7 +124,7 @@ class Writer {
         const std::string base = dir_ + "/" + prefix_ + "_" + tag;
         if (format_ == "csv" || format_ == "all") csv(grid, base + ".csv", t, snap);
         if (format_ == "vtk" || format_ == "all") vtk(grid, base + ".vtk", t, snap);
-        if (format_ == "ppm" || format_ == "all") ppm(grid, base + "_grains.ppm");
+        if (format_ == "ppm" || format_ == "all") ppm(grid, base);
     }
 
   private:
@@ -181,22 +181,56 @@ class Writer {
         }
     }
 
-    void ppm(const GPKGrid& grid, const std::string& path) const {
-        const int z0 = grid.Nz / 2, W = grid.Nx, H = grid.Ny;
-        std::vector<unsigned char> rgb(size_t(W) * H * 3);
-        for (int y = 0; y < H; ++y)
-            for (int x = 0; x < W; ++x) {
-                const int i = grid.idx(z0, y, x);
-                const uint32_t hsh = uint32_t(grid.phi[i]) * 2654435761u;
-                const float a = grid.mask[i] ? grid.alpha[i] : 1.f;
-                unsigned char* p = &rgb[(size_t(y) * W + x) * 3];
-                p[0] = (unsigned char)(a * float(64 + (hsh >> 8 & 191)));
-                p[1] = (unsigned char)(a * float(64 + (hsh >> 16 & 191)));
-                p[2] = (unsigned char)(a * float(64 + (hsh >> 24 & 191)));
-            }
+    // Mid-plane (z = Nz/2) images, utils/gpk_io.hpp:77-101 and
+    // utils/ppm_writer.hpp: grain colours from a multiplicative label hash,
+    // alpha-blended with the competitor colour at interface cells; the
+    // composition slice through a piecewise-linear purple-teal-yellow map,
+    // written only when c varies on the slice.
+    static unsigned char unit_to_byte(float v) {
+        return static_cast<unsigned char>(std::min(std::max(v, 0.f), 1.f) * 255.f + 0.5f);
+    }
+    static void label_colour(int32_t label, unsigned char rgb[3]) {
+        const uint32_t h = static_cast<uint32_t>(label) * 2654435761u;
+        for (int k = 0; k < 3; ++k) rgb[k] = static_cast<unsigned char>(80 + ((h >> (8 * k)) & 0x7F));
+    }
+    static void write_p6(const std::string& path, int W, int H, const std::vector<unsigned char>& rgb) {
         std::ofstream f(path, std::ios::binary);
         f << "P6\n" << W << " " << H << "\n255\n";
         f.write(reinterpret_cast<const char*>(rgb.data()), std::streamsize(rgb.size()));
+    }
+
+    void ppm(const GPKGrid& grid, const std::string& base) const {
+        const int W = grid.Nx, H = grid.Ny;
+        const size_t plane = size_t(W) * H, off = size_t(grid.Nz / 2) * plane;
+        std::vector<unsigned char> rgb(plane * 3);
+        for (size_t p = 0; p < plane; ++p) {
+            const size_t i = off + p;
+            unsigned char* px = &rgb[3 * p];
+            label_colour(grid.phi[i], px);
+            if (grid.mask[i] && grid.phi_comp[i] != grid.phi[i]) {
+                unsigned char other[3];
+                label_colour(grid.phi_comp[i], other);
+                const float t = std::min(std::max(grid.alpha[i], 0.f), 1.f);
+                for (int k = 0; k < 3; ++k)
+                    px[k] = unit_to_byte(t * (px[k] / 255.f) + (1.f - t) * (other[k] / 255.f));
+            }
+        }
+        write_p6(base + "_grains.ppm", W, H, rgb);
+
+        const auto c0 = grid.c.begin() + std::ptrdiff_t(off);
+        const float lo = *std::min_element(c0, c0 + std::ptrdiff_t(plane));
+        const float hi = *std::max_element(c0, c0 + std::ptrdiff_t(plane));
+        if (!(hi > lo)) return;
+        const float span = hi - lo;
+        for (size_t p = 0; p < plane; ++p) {
+            const float t = std::min(std::max((grid.c[off + p] - lo) / span, 0.f), 1.f);
+            const bool low = t < 0.5f;
+            const float u = t - 0.5f;
+            rgb[3 * p + 0] = unit_to_byte(low ? 0.27f + 0.1f * t : 0.2f + 1.6f * u);
+            rgb[3 * p + 1] = unit_to_byte(low ? 0.4f * t : 0.2f + 1.2f * u);
+            rgb[3 * p + 2] = unit_to_byte(low ? 0.5f - t : 0.5f - u);
+        }
+        write_p6(base + "_comp.ppm", W, H, rgb);
     }
 };
 

@@ -18,6 +18,10 @@ Fixtures (inputs and expected outputs only):
   voronoi_*.npz      initial labels and orientations written by the reference's
                      own CLI (demos/gpk.cpp voronoi_init, 51-189) for small decks,
                      run from oracle/_ref/gpk
+  cli_*.npz          a deck (text, output dir templated as {out}) and every
+                     snapshot file the reference CLI wrote for it (CSV, VTK and
+                     PPM bytes), demos/gpk.cpp run_grain_growth (195-282) and
+                     run_solidification (288-391)
 """
 import argparse
 import os
@@ -106,6 +110,128 @@ def voronoi(cli, nz, ny, nx, ng, seed):
                         dims=np.array([nz, ny, nx, ng, seed], np.int64))
 
 
+CLI_DECKS = {
+    "cli_grain_growth_2d": """[run]
+model = grain_growth
+[grid]
+Nz = 1
+Ny = 64
+Nx = 48
+dx = 1.0e-6
+[time]
+t_end = 2.0
+n_snaps = 2
+[thermo]
+M0 = 1.0e-10
+sigma0 = 0.5
+[grain_growth]
+n_grains = 30
+seed = 7
+[geometry]
+sigma_cells = 1.5
+[output]
+format = all
+dir = {out}
+write_initial = true
+""",
+    "cli_solidification_2d": """[run]
+model = solidification
+[grid]
+Nz = 1
+Ny = 64
+Nx = 64
+dx = 0.5e-6
+[time]
+t_end = 2.0e-4
+n_snaps = 2
+[thermo]
+M0 = 5.0e-8
+sigma0 = 0.20
+eps4 = 0.04
+D = 3.0e-9
+m_liq = -500.0
+k_part = 0.3
+T_melt = 933.0
+L = 1.0e8
+stochastic = false
+solute_rejection = true
+[initial]
+c0 = 0.02
+undercooling = 5.0
+seed_radius = 4
+[output]
+format = all
+dir = {out}
+write_initial = true
+""",
+    "cli_solidification_3d": """[run]
+model = solidification
+[grid]
+Nz = 24
+Ny = 24
+Nx = 24
+dx = 0.5e-6
+[time]
+t_end = 1.0e-4
+n_snaps = 2
+[thermo]
+M0 = 5.0e-8
+sigma0 = 0.20
+eps4 = 0.04
+D = 3.0e-9
+m_liq = -500.0
+k_part = 0.3
+T_melt = 933.0
+L = 1.0e8
+solute_rejection = true
+[initial]
+c0 = 0.02
+undercooling = 5.0
+seed_radius = 4
+[output]
+format = all
+dir = {out}
+prefix = s3
+""",
+    "cli_grain_growth_3d": """[run]
+model = grain_growth
+[grid]
+Nz = 16
+Ny = 20
+Nx = 24
+dx = 1.0e-6
+[time]
+t_end = 0.3
+n_snaps = 2
+[thermo]
+M0 = 1.0e-10
+sigma0 = 0.5
+[grain_growth]
+n_grains = 40
+seed = 3
+[output]
+format = csv
+dir = {out}
+prefix = g3
+""",
+}
+
+
+def cli_snapshots(cli, name, deck):
+    with tempfile.TemporaryDirectory() as tmp:
+        out = os.path.join(tmp, "out")
+        path = os.path.join(tmp, "deck.ini")
+        with open(path, "w") as f:
+            f.write(deck.format(out=out))
+        subprocess.check_call([cli, path], stdout=subprocess.DEVNULL, cwd=tmp)
+        files = sorted(os.listdir(out))
+        blobs = [np.frombuffer(open(os.path.join(out, fn), "rb").read(), np.uint8) for fn in files]
+    arrays = {"deck": np.array(deck), "files": np.array(files)}
+    for k, b in enumerate(blobs):
+        arrays["f%d" % k] = b
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
@@ -116,6 +242,8 @@ def main():
     cli = os.path.join(REPO, "oracle", "_ref", "gpk")
     for dims in ((1, 64, 48, 30, 7), (1, 96, 96, 200, 42), (16, 20, 24, 40, 3), (24, 24, 24, 60, 42)):
         voronoi(cli, *dims)
+    for name, deck in CLI_DECKS.items():
+        cli_snapshots(cli, name, deck)
 
 
 if __name__ == "__main__":

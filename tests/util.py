@@ -6,6 +6,7 @@ import numpy as np
 from gpk_amd import _abi
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 FIELDS_ALL = ("phi", "alpha", "g", "mask", "phi_comp", "kappa", "nx", "ny", "nz", "vn", "g_comp", "c")
 
