@@ -58,6 +58,8 @@ def lib():
     L.gpk_interface_fraction.argtypes = [H, C.POINTER(C.c_float)]
     L.gpk_solute_sum.argtypes = [H, C.POINTER(C.c_double)]
     L.gpk_device_bytes.argtypes = [H, C.POINTER(C.c_size_t)]
+    L.gpk_velocity_field.argtypes = [H, C.POINTER(C.c_double), vp]
+    L.gpk_velocity_adjoint.argtypes = [H, C.POINTER(C.c_double), vp, C.POINTER(C.c_double)]
     L.gpk_topology_simple_move.argtypes = [H, C.c_int, C.c_int32, C.POINTER(C.c_int)]
     L.gpk_voronoi_init.argtypes = [H, C.c_int, C.c_uint32]
     L.gpk_voronoi_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, vp, vp]
@@ -84,4 +86,5 @@ EXPORTS = (
     "gpk_phase_volume", "gpk_n_grains", "gpk_mean_grain_radius", "gpk_solid_fraction",
     "gpk_interface_fraction", "gpk_solute_sum", "gpk_device_bytes", "gpk_topology_simple_move",
     "gpk_voronoi_init", "gpk_voronoi_host", "gpk_sync", "gpk_set_profiling", "gpk_kernel_timing",
+    "gpk_velocity_field", "gpk_velocity_adjoint",
 )

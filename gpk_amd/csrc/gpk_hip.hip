@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "gpk_kernels.hpp"
+#include "gpk_adjoint.hpp"
 
 using namespace gpkd;
 
@@ -404,6 +405,8 @@ struct EngineBase {
     virtual void set_profiling(int on) = 0;
     virtual void timing(int which, double* ms, int64_t* n, double* bytes) = 0;
     virtual int64_t steps() const = 0;
+    virtual void velocity_field(const double theta[3], void* vout) = 0;
+    virtual void velocity_adjoint(const double theta[3], const void* gvel, double grad[3]) = 0;
 };
 
 template <class R, bool kDirect>
@@ -836,17 +839,8 @@ struct Engine final : EngineBase {
         return hs->nsel;
     }
 
-    // ---- the step (src/gpk_simulation.cpp:31-54) ----
-    void step(float dt_req, float cap, gpk_step_stats* st) override {
-        const int nb = band_compact(bandA, true);
-        geo_band = nb;
-        // geometry + competitor + constitutive velocity
-        GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
-        if (nb > 0) {
-            if (profiling) record_geo_begin();
-            launch_geometry(nb, gp);
-            if (profiling) record_geo_end(nb);
-        }
+    // pair projection of vraw into vn (src/gpk_physics.cpp:152-193), vmax on the side
+    void project(int nb) {
         k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
         // the tiled projection packs (low, high, orientation) in 32 bits: labels < 2^15
         if (nb > 0 && nlab <= 32768 && proj_r == 3) {
@@ -874,6 +868,96 @@ struct Engine final : EngineBase {
                     phi.as<int32_t>(), phi_comp.as<int32_t>(), projw.as<double>(), vraw.as<R>(),
                     vn.as<R>(), reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
         }
+    }
+
+    // ---- velocity chain for the adjoint: geometry + law + projection on the
+    // current labels with (eps4, eps6, M0) = theta; leaves the derived band
+    // fields (mask, normals, curvature, phi_comp, vraw, vn) for these labels ----
+    int velocity_chain(const double theta[3]) {
+        if (model.kind != GPK_MODEL_ISOTROPIC && model.kind != GPK_MODEL_ANISO_GROWTH)
+            throw GpkError(GPK_E_UNSUPPORTED_MODEL,
+                           "velocity adjoint: isotropic or anisotropic-growth models only");
+        const gpk_model_desc saved = model;
+        model.eps4 = float(theta[0]);
+        model.eps6 = float(theta[1]);
+        model.M0 = float(theta[2]);
+        int nb = 0;
+        try {
+            nb = band_compact(bandA, true);
+            geo_band = nb;
+            if (nb > 0) launch_geometry(nb, GeomParams{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells});
+            project(nb);
+        } catch (...) {
+            model = saved;
+            throw;
+        }
+        model = saved;
+        return nb;
+    }
+
+    void velocity_field(const double theta[3], void* vout) override {
+        const int nb = velocity_chain(theta);
+        k_fill_r<R><<<gsblk(N, 256), 256, 0, stream>>>(N, static_cast<R*>(vout), R(0));
+        if (nb > 0)
+            k_scatter_band<R><<<nblk(nb, 256), 256, 0, stream>>>(nb, bandA.as<int>(), vn.as<R>(),
+                                                                 static_cast<R*>(vout));
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    DevBuf adj_h, adj_part, adj_out;
+    void velocity_adjoint(const double theta[3], const void* gvel, double grad[3]) override {
+        const double th[3] = {theta[0], theta[1], theta[2]};
+        const int nb = velocity_chain(th);
+        grad[0] = grad[1] = grad[2] = 0.0;
+        if (nb == 0) {
+            HIPCHK(hipStreamSynchronize(stream));
+            return;
+        }
+        const int nblocks = nblk(nb, 256);
+        if (adj_h.bytes < size_t(nb) * 8) adj_h.alloc(size_t(nb) * 8);
+        if (adj_part.bytes < size_t(nblocks) * 24) adj_part.alloc(size_t(nblocks) * 24);
+        if (!adj_out.p) adj_out.alloc(24);
+        gpk_model_desc m = model;
+        m.eps4 = float(theta[0]);
+        m.eps6 = float(theta[1]);
+        m.M0 = float(theta[2]);
+        const R* gv = static_cast<const R*>(gvel);
+        if (dim3) {
+            k_adj_h<R, 3><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
+                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                                                        projw.as<double>(), gv, adj_h.as<double>());
+            k_adj_terms<R, 3><<<nblocks, 256, 0, stream>>>(
+                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
+                phi_comp.as<int32_t>(), projw.as<double>(), gv, adj_h.as<double>(), nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, adj_part.as<double>());
+        } else {
+            k_adj_h<R, 2><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
+                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                                                        projw.as<double>(), gv, adj_h.as<double>());
+            k_adj_terms<R, 2><<<nblocks, 256, 0, stream>>>(
+                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
+                phi_comp.as<int32_t>(), projw.as<double>(), gv, adj_h.as<double>(), nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, adj_part.as<double>());
+        }
+        k_adj_final<<<1, 256, 0, stream>>>(nblocks, adj_part.as<double>(), adj_out.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad, adj_out.p, 24, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // ---- the step (src/gpk_simulation.cpp:31-54) ----
+    void step(float dt_req, float cap, gpk_step_stats* st) override {
+        const int nb = band_compact(bandA, true);
+        geo_band = nb;
+        // geometry + competitor + constitutive velocity
+        GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
+        if (nb > 0) {
+            if (profiling) record_geo_begin();
+            launch_geometry(nb, gp);
+            if (profiling) record_geo_end(nb);
+        }
+        project(nb);
         k_dt<R><<<1, 1, 0, stream>>>(ds(), R(dt_req), R(cap), dx_f, D_f, dim3);
         const bool partition = (model.flags & GPK_FLAG_SOLUTE_REJECTION) != 0;
         if (partition)
@@ -1397,6 +1481,22 @@ int gpk_run(gpk_handle* h, double t_end, float dt, int* steps_taken) {
             ++n;
         }
         if (steps_taken) *steps_taken = n;
+    });
+}
+
+int gpk_velocity_field(gpk_handle* h, const double theta[3], void* vout_dev) {
+    return guarded([&] {
+        NEED_H;
+        if (!theta || !vout_dev) throw GpkError(GPK_E_INVALID, "null theta or output");
+        h->eng->velocity_field(theta, vout_dev);
+    });
+}
+
+int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]) {
+    return guarded([&] {
+        NEED_H;
+        if (!theta || !gvel_dev || !grad) throw GpkError(GPK_E_INVALID, "null theta, gradient or output");
+        h->eng->velocity_adjoint(theta, gvel_dev, grad);
     });
 }
 

@@ -30,6 +30,7 @@ class GPKGrid:
         self.dx = float(dx)
         self.N = self.Nz * self.Ny * self.Nx
         self.precision = precision
+        self.device = int(device)
         self.real = np.float64 if precision == "fp64" else np.float32
         self._model = thermo
         gd = _abi.GridDesc()
@@ -62,6 +63,10 @@ class GPKGrid:
 
     def _dtype(self, kind):
         return {"i32": np.int32, "u8": np.uint8, "real": self.real}[kind]
+
+    @property
+    def shape(self):
+        return self._shape()
 
     def get(self, name):
         fid, kind = _abi.FIELDS[_FIELD_ALIASES.get(name, name)]
@@ -172,6 +177,19 @@ class GPKGrid:
         v = C.c_double()
         check(lib().gpk_solute_sum(self._h, C.byref(v)))
         return v.value
+
+    def velocity_field_ptr(self, theta, vout_ptr):
+        """Projected velocity for theta = (eps4, eps6, M0) into a device buffer
+        of N reals (gpk_velocity_field)."""
+        th = (C.c_double * 3)(*[float(t) for t in theta])
+        check(lib().gpk_velocity_field(self._h, th, C.c_void_p(vout_ptr)))
+
+    def velocity_adjoint_ptr(self, theta, gvel_ptr):
+        """dL/dtheta from dL/dv in a device buffer of N reals (gpk_velocity_adjoint)."""
+        th = (C.c_double * 3)(*[float(t) for t in theta])
+        out = (C.c_double * 3)()
+        check(lib().gpk_velocity_adjoint(self._h, th, C.c_void_p(gvel_ptr), out))
+        return (out[0], out[1], out[2])
 
     def device_bytes(self):
         v = C.c_size_t()

@@ -233,6 +233,25 @@ int gpk_set_profiling(gpk_handle* h, int enable);
 int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launches,
                       double* flops_per_launch);
 
+/* Velocity chain and its adjoint (no reference counterpart: the reference
+ * has no derivatives).  theta = {eps4, eps6, M0} replaces those model
+ * parameters for the call only; models ISOTROPIC and ANISO_GROWTH.
+ *
+ * gpk_velocity_field recomputes the band, geometry and constitutive velocity
+ * on the current labels (src/gpk_physics.cpp:67-148), applies the pair
+ * projection (src/gpk_physics.cpp:152-193) and writes the projected velocity
+ * of every cell into vout_dev (device pointer, N values of the handle's real
+ * type: float for STRICT32/FP32, double for FP64; 0 off the band).
+ *
+ * gpk_velocity_adjoint takes gvel_dev = dL/dv (device, N values of the real
+ * type, read on band cells only) and returns dL/dtheta in grad[3] (FP64,
+ * fixed-order reduction): the transposed projection applied to gvel, times
+ * dv_raw/dtheta.  Normals, curvature and projection weights are recomputed.
+ * Both calls return after the handle's stream has drained; the caller makes
+ * sure gvel_dev is complete (and vout_dev unused) before the call. */
+int gpk_velocity_field(gpk_handle* h, const double theta[3], void* vout_dev);
+int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]);
+
 #ifdef __cplusplus
 }
 #endif
