@@ -62,6 +62,13 @@ class GridDesc(C.Structure):
     ]
 
 
+class BatchStats(C.Structure):
+    """gpk_batch_stats (include/gpk_c.h)."""
+    _fields_ = [("band_cells", C.c_int64), ("accepted", C.c_int64), ("exchanges", C.c_int64),
+                ("proposals", C.c_int64), ("instances", C.c_int32), ("workgroup_size", C.c_int32),
+                ("workgroups", C.c_int32)]
+
+
 class StepStats(C.Structure):
     _fields_ = [
         ("band_cells", C.c_int64), ("candidates", C.c_int64), ("accepted", C.c_int64),

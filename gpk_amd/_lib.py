@@ -58,6 +58,17 @@ def lib():
     L.gpk_interface_fraction.argtypes = [H, C.POINTER(C.c_float)]
     L.gpk_solute_sum.argtypes = [H, C.POINTER(C.c_double)]
     L.gpk_device_bytes.argtypes = [H, C.POINTER(C.c_size_t)]
+    L.gpk_batch_create.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_float,
+                                   C.c_float, C.c_float, C.c_int, C.c_int, C.POINTER(_abi.ModelDesc),
+                                   C.POINTER(H)]
+    L.gpk_batch_destroy.argtypes = [H]
+    L.gpk_batch_upload.argtypes = [H, C.c_int, vp, C.c_size_t]
+    L.gpk_batch_download.argtypes = [H, C.c_int, vp, C.c_size_t]
+    L.gpk_batch_step.argtypes = [H, C.c_int, C.c_float, C.c_float]
+    L.gpk_batch_stats_get.argtypes = [H, C.POINTER(_abi.BatchStats)]
+    L.gpk_batch_instances.argtypes = [H, vp, vp, vp]
+    L.gpk_batch_autotune.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.gpk_batch_set_launch.argtypes = [H, C.c_int, C.c_int]
     L.gpk_velocity_field.argtypes = [H, C.POINTER(C.c_double), vp]
     L.gpk_velocity_adjoint.argtypes = [H, C.POINTER(C.c_double), vp, C.POINTER(C.c_double)]
     L.gpk_topology_simple_move.argtypes = [H, C.c_int, C.c_int32, C.POINTER(C.c_int)]
@@ -87,4 +98,6 @@ EXPORTS = (
     "gpk_interface_fraction", "gpk_solute_sum", "gpk_device_bytes", "gpk_topology_simple_move",
     "gpk_voronoi_init", "gpk_voronoi_host", "gpk_sync", "gpk_set_profiling", "gpk_kernel_timing",
     "gpk_velocity_field", "gpk_velocity_adjoint",
+    "gpk_batch_create", "gpk_batch_destroy", "gpk_batch_upload", "gpk_batch_download", "gpk_batch_step",
+    "gpk_batch_stats_get", "gpk_batch_instances", "gpk_batch_autotune", "gpk_batch_set_launch",
 )

@@ -252,6 +252,46 @@ int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launche
 int gpk_velocity_field(gpk_handle* h, const double theta[3], void* vout_dev);
 int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]);
 
+/* ---- Batched 2-D instances ------------------------------------------------
+ * No counterpart in the reference, which steps the grids of a sweep one
+ * after another (demos/gpk.cpp:1121-1259).  A batch holds n_inst 2-D grids
+ * (Ny[b] x Nx[b], shared dx, sigma, precision and model) packed back to back
+ * in instance order; one persistent kernel advances every instance by one
+ * GPKGrid::step per launch, each with its own dt and time.  Fields are
+ * uploaded / downloaded packed (instance 0's cells first, each instance in
+ * its own z-y-x order) for GPK_FIELD_PHI, ALPHA, G, C, T, MASK, PHI_COMP,
+ * G_COMP.  Scope: D = 0, no grain-network topology test, no solute rejection,
+ * no noise (e.g. ANISO_OSTWALD with c = c_sat, TENSOR_ANISO); labels in
+ * [0, 1024).  Other models return GPK_E_UNSUPPORTED_MODEL. */
+typedef struct gpk_batch gpk_batch;
+
+typedef struct gpk_batch_stats {
+    int64_t band_cells;   /* sum over instances and steps of |A| */
+    int64_t accepted;     /* accepted flips */
+    int64_t exchanges;    /* remap exchanges */
+    int64_t proposals;    /* remap proposals */
+    int32_t instances;
+    int32_t workgroup_size, workgroups;   /* launch shape in use */
+} gpk_batch_stats;
+
+int gpk_batch_create(int n_inst, const int* Ny, const int* Nx, float dx, float D, float gaussian_sigma,
+                     int precision, int device, const gpk_model_desc* model, gpk_batch** out);
+int gpk_batch_destroy(gpk_batch* h);
+int gpk_batch_upload(gpk_batch* h, int field, const void* host, size_t bytes);
+int gpk_batch_download(gpk_batch* h, int field, void* host, size_t bytes);
+/* nsteps steps of every instance (dt < 0: each instance's stable dt; dt_cap
+ * caps it); a step is one replay of a captured hipGraph. */
+int gpk_batch_step(gpk_batch* h, int nsteps, float dt, float dt_cap);
+int gpk_batch_stats_get(gpk_batch* h, gpk_batch_stats* stats);
+/* Per instance (arrays of n_inst, any may be NULL): simulation time, device
+ * duration of its last step in ms, band size of its last step. */
+int gpk_batch_instances(gpk_batch* h, double* time, double* last_step_ms, int* band_cells);
+/* Autotuner: times two steps for each workgroup size (256, 512, 1024) and
+ * workgroup count (1, 2, 4, 8 x 256-thread slots per CU) on a copy of the
+ * state, restores the state, keeps the fastest shape and returns it. */
+int gpk_batch_autotune(gpk_batch* h, int* workgroup_size, int* workgroups, double* step_ms);
+int gpk_batch_set_launch(gpk_batch* h, int workgroup_size, int workgroups);
+
 #ifdef __cplusplus
 }
 #endif
