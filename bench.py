@@ -14,6 +14,14 @@ bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
+`--workload batch` runs the lamellar sweep: 512 2-D instances, side
+~ U_int[64, 384], stripe width t ~ U_int[6, 20], period round(r t) with
+r ~ U[2, 12], one-cell seeded edge perturbations (numpy default_rng(0)),
+AnisoOstwaldThermo(M0 1, sigma0 1, bxx 1, byy 0.5, bzz 1, c_sat 0.1,
+Omega 1e8) at c = c_sat (pure capillary flow), D = 0, sigma 1.5; 300 steps
+in FP32 and FP64 through the batched persistent kernel after the autotuner
+has picked its launch shape.  value = instance-steps/s.
+
 `--workload solidification` instead runs the solute-coupled case: 128^3,
 SolidificationThermo(M0 5e-8, sigma0 0.2, eps4 0.02, m_liq -500, k 0.14,
 T_melt 933, L 1e8) with partitioned transport (solute_rejection), D = 1e-9,
@@ -198,9 +206,96 @@ def main_solidification(args):
     print(json.dumps(out))
 
 
+def batch_model():
+    from gpk_amd import _abi
+    return _abi.aniso_ostwald(1.0, 1.0, 1.0, 0.5, 1.0, float(np.float32(0.1)), 1e8)
+
+
+def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20):
+    from gpk_amd.batch import GPKBatch
+    B = GPKBatch(shapes, 1.0, batch_model(), 0.0, sigma, precision=prec)
+    B.set("phi", phis)
+    B.set("c", [np.full(s, np.float32(0.1), np.float32) for s in shapes])
+    t0 = time.perf_counter()
+    wg, nwg, tune_ms = B.autotune()
+    t_tune = time.perf_counter() - t0
+    B.step(1)  # graph capture + warm-up step
+    t0 = time.perf_counter()
+    B.step(steps)
+    wall = time.perf_counter() - t0
+    st = B.stats()
+    phi = B.get("phi")
+    per = []
+    for _ in range(sample_steps):
+        B.step(1)
+        per.append(B.instances()[1])
+    per = np.concatenate(per)
+    B.close()
+    n = len(shapes)
+    return phi, {
+        "precision": prec, "instances": n, "steps": steps, "seconds": wall,
+        "ms_per_step": 1e3 * wall / steps, "instance_steps_per_s": n * steps / wall,
+        "cell_updates_per_s": st["band_cells"] * steps / (steps + 1) / wall,
+        "accepted": int(st["accepted"]), "exchanges": int(st["exchanges"]),
+        "instance_step_ms_p50": float(np.percentile(per, 50)),
+        "instance_step_ms_p99": float(np.percentile(per, 99)),
+        "autotune": {"workgroup_size": wg, "workgroups": nwg, "step_ms": tune_ms, "seconds": t_tune},
+    }
+
+
+def cpu_batch_baseline(shapes, phis, sigma, n_inst=32, steps=30):
+    """Reference GPKGrid steps (compiled reference sources) on a sample of instances."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref_available() else "port"
+    wall = 0.0
+    for s, p in list(zip(shapes, phis))[:n_inst]:
+        if kind == "reference":
+            R = O.RefGrid(1, s[0], s[1], 1.0, batch_model(), 0.0, sigma)
+        else:
+            R = O.OracleGrid(1, s[0], s[1], 1.0, batch_model(), 0.0, sigma, "strict32")
+        R.set("phi", p)
+        R.set("c", np.full(s, np.float32(0.1), np.float32))
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            R.step()
+        wall += time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": n_inst * steps / wall, "unit": "instance-steps/s", "cores": threads, "kind": kind,
+            "seconds": wall,
+            "sample": "%d reference steps of each of the first %d instances, run one after another" %
+                      (steps, n_inst)}
+
+
+def main_batch(args):
+    from gpk_amd.batch import lamellar_sweep
+    n = 512
+    steps = args.steps if args.steps != 20 else 300
+    shapes, phis, _ = lamellar_sweep(n, seed=0)
+    p32, r32 = run_batch("fp32", shapes, phis, steps, args.sigma)
+    p64, r64 = run_batch("fp64", shapes, phis, steps, args.sigma)
+    diff = sum(int(np.count_nonzero(a != b)) for a, b in zip(p32, p64))
+    r32["label_disagreement_vs_fp64"] = diff / float(sum(a.size for a in p32))
+    cpu = None if args.no_cpu else cpu_batch_baseline(shapes, phis, args.sigma)
+    out = {
+        "metric": "instance-steps/s, 512 batched 2-D lamellar instances",
+        "value": r32["instance_steps_per_s"], "unit": "instance-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded lamellar sweep)",
+        "config": {"workload": "512 2-D instances, side U[64,384], lamellae t U[6,20], period r t, "
+                               "r U[2,12], AnisoOstwald c = c_sat, %d steps" % steps,
+                   "instances": n, "cells": int(sum(a.size for a in p32)), "precision": "fp32",
+                   "parallelism": "single GPU"},
+        "cpu_baseline": cpu, "fp32": r32, "fp64": r64,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = r32["instance_steps_per_s"] / cpu["value"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["grain_growth", "solidification"], default="grain_growth")
+    ap.add_argument("--workload", choices=["grain_growth", "solidification", "batch"],
+                    default="grain_growth")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -217,6 +312,9 @@ def main():
         sys.exit(2)
     if args.workload == "solidification":
         main_solidification(args)
+        return
+    if args.workload == "batch":
+        main_batch(args)
         return
 
     G64, r64 = run_gpu("fp64", args.size, args.grains, args.sigma, args.warmup, args.steps)
