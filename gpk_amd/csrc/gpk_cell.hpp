@@ -68,36 +68,15 @@ __device__ __forceinline__ void row_sum(const LutRef<R>& lr, uint64_t m, R& R0, 
     row_sum<R>(lr.L, lr.t, m, R0, R1, R2);
 }
 
-// normals, curvature (src/gpk_geometry.cpp:136-151), competitor (154-182) and
-// the constitutive velocity of band cell i (slot a) from its stencil sums
+// competitor (src/gpk_geometry.cpp:154-182) and constitutive velocity
+// (src/gpk_physics.cpp:97-146) of band cell i (slot a) from its normal and
+// curvature
 template <class R, int DIM, class CubeAt>
-__device__ __forceinline__ void finish_cell(
-    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
-    int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
-    const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
-    const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
-    R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
-    R* __restrict__ g_comp) {
-    const R dx = R(gp.dx_f);
-    const R inv_dx = R(1) / dx;
-    const R inv_dx2 = inv_dx * inv_dx;
-    const R klim = R(1) / (R(gp.sigma_cells) * dx);
-    Ix *= inv_dx; Iy *= inv_dx; Iz *= inv_dx;
-    Ixx *= inv_dx2; Iyy *= inv_dx2; Izz *= inv_dx2;
-    Ixy *= inv_dx2; Ixz *= inv_dx2; Iyz *= inv_dx2;
-    const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
-    R nxv = 0, nyv = 0, nzv = 0, kap = 0;
-    if (gsq * dx * dx > Eps<R>::grad) {
-        const R gr = dsqrt(gsq);
-        nxv = -Ix / gr; nyv = -Iy / gr; nzv = -Iz / gr;
-        const R trH = Ixx + Iyy + Izz;
-        const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
-                    + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
-        R kv = (gHg - gsq * trH) / (gsq * gr);
-        if (dfabs(kv * dx) < Eps<R>::kappa) kv = R(0);
-        kap = kv < -klim ? -klim : (klim < kv ? klim : kv);
-    }
-    onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
+__device__ __forceinline__ void comp_vel_cell(
+    const Dims& d, const gpk_model_desc& model, int i, int a, int z0, int y0, int x0, int32_t res,
+    R nxv, R nyv, R nzv, R kap, CubeAt cube_at, const int32_t* __restrict__ phi,
+    const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T, R* __restrict__ vraw,
+    int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     (void)cube_at;
 
     // competitor: most-supported unlike face label, ties by outward-normal
@@ -147,6 +126,40 @@ __device__ __forceinline__ void finish_cell(
     const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
     const R mob = R(model.M0);
     vraw[a] = mob * (dG - sig * kap);
+}
+
+// normals, curvature (src/gpk_geometry.cpp:136-151), competitor (154-182) and
+// the constitutive velocity of band cell i (slot a) from its stencil sums
+template <class R, int DIM, class CubeAt>
+__device__ __forceinline__ void finish_cell(
+    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
+    int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
+    const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
+    const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
+    R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
+    R* __restrict__ g_comp) {
+    const R dx = R(gp.dx_f);
+    const R inv_dx = R(1) / dx;
+    const R inv_dx2 = inv_dx * inv_dx;
+    const R klim = R(1) / (R(gp.sigma_cells) * dx);
+    Ix *= inv_dx; Iy *= inv_dx; Iz *= inv_dx;
+    Ixx *= inv_dx2; Iyy *= inv_dx2; Izz *= inv_dx2;
+    Ixy *= inv_dx2; Ixz *= inv_dx2; Iyz *= inv_dx2;
+    const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
+    R nxv = 0, nyv = 0, nzv = 0, kap = 0;
+    if (gsq * dx * dx > Eps<R>::grad) {
+        const R gr = dsqrt(gsq);
+        nxv = -Ix / gr; nyv = -Iy / gr; nzv = -Iz / gr;
+        const R trH = Ixx + Iyy + Izz;
+        const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
+                    + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
+        R kv = (gHg - gsq * trH) / (gsq * gr);
+        if (dfabs(kv * dx) < Eps<R>::kappa) kv = R(0);
+        kap = kv < -klim ? -klim : (klim < kv ? klim : kv);
+    }
+    onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
+    comp_vel_cell<R, DIM>(d, model, i, a, z0, y0, x0, res, nxv, nyv, nzv, kap, cube_at, phi, g, c, T,
+                          vraw, phi_comp, g_comp);
 }
 
 

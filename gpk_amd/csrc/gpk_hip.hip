@@ -26,6 +26,7 @@
 
 #include "gpk_kernels.hpp"
 #include "gpk_adjoint.hpp"
+#include "gpk_broad.hpp"
 #include "gpk_host.hpp"
 
 using namespace gpkd;
@@ -271,6 +272,11 @@ struct Engine final : EngineBase {
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg;
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
+    // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
+    // flags / list of the interface labels, hipcub scratch for the label list
+    bool broad = false;
+    DevBuf kw, lflag, llist, lcub;
+    std::vector<int32_t> hlabels;
     DevBuf scal, dsum, tagctr, minmax, noise_amp;
     DevBuf cub_tmp;
     size_t cub_bytes = 0;
@@ -300,6 +306,7 @@ struct Engine final : EngineBase {
         remap_sigma = std::max(Nz > 1 ? std::clamp(0.125f * sigma_f, 0.70f, 1.00f) : 0.70f, 0.5f);
         proj_r = std::max(1, static_cast<int>(std::ceil(3.f * vel_sigma)));
         remap_r = std::max(1, static_cast<int>(std::ceil(3.f * remap_sigma)));
+        broad = dim3 && W.rg > 18;   // src/gpk_grid.cpp:44
         allocate();
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -326,7 +333,8 @@ struct Engine final : EngineBase {
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
                                &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
-                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp};
+                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag,
+                               &llist, &lcub};
         for (auto* b : all) t += b->bytes;
         return t;
     }
@@ -375,12 +383,24 @@ struct Engine final : EngineBase {
         HIPCHK(hipMemcpy(wt0.p, w0.data(), Wd * rs, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(wt1.p, w1.data(), Wd * rs, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(wt2.p, w2.data(), Wd * rs, hipMemcpyHostToDevice));
+        if (broad) {
+            // plain Gaussian taps: w0 of the geometry stencil is the Gaussian
+            // padded by two zeros on each side
+            const int nk = 2 * W.radius + 1;
+            kw.alloc(nk * rs);
+            HIPCHK(hipMemcpy(kw.p, w0.data() + (W.rg - W.radius), nk * rs, hipMemcpyHostToDevice));
+        }
         // direct-stencil product table, (gx*gy)*gz in the reference's order
-        if (kDirect) {
+        if (broad) {
+            tab.alloc(16);
+        } else if (kDirect) {
             const std::vector<R> t = direct_table<R>(w0, w1, w2, dim3 != 0);
             tab.alloc(t.size() * rs);
             HIPCHK(hipMemcpy(tab.p, t.data(), t.size() * rs, hipMemcpyHostToDevice));
         } else {
+            if (Wd > 64)
+                throw GpkError(GPK_E_INVALID, "fp32/fp64 geometry needs a stencil of at most 64 cells "
+                                              "(radius <= 31); use strict32 for wider 2-D stencils");
             std::vector<R> t;
             lutd = make_row_lut<R>(w0, w1, w2, t);
             tab.alloc(t.size() * rs);
@@ -600,7 +620,48 @@ struct Engine final : EngineBase {
         default: return false;
         }
     }
+    // src/gpk_geometry.cpp:185-390: per interface label, ascending
+    void broad_geometry(int nb) {
+        const int L = std::max(nlab, 1);
+        if (lflag.bytes < size_t(L)) { lflag.alloc(size_t(L)); llist.alloc(size_t(L) * 4); }
+        k_fill_u8<<<gsblk(L, 256), 256, 0, stream>>>(L, lflag.as<uint8_t>(), 0);
+        k_broad_flag_labels<<<nblk(nb, 256), 256, 0, stream>>>(nb, bandA.as<int>(), phi.as<int32_t>(),
+                                                               lflag.as<uint8_t>());
+        hipcub::CountingInputIterator<int> it(0);
+        size_t tb = 0;
+        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, lflag.as<uint8_t>(), llist.as<int>(),
+                                             &ds()->nsel, L, stream));
+        if (lcub.bytes < tb) lcub.alloc(tb);
+        HIPCHK(hipcub::DeviceSelect::Flagged(lcub.p, tb, it, lflag.as<uint8_t>(), llist.as<int>(),
+                                             &ds()->nsel, L, stream));
+        pull_scalars();
+        const int ng = hs->nsel;
+        hlabels.resize(ng);
+        if (ng > 0) {
+            HIPCHK(hipMemcpyAsync(hlabels.data(), llist.p, size_t(ng) * 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        const int kr = W.radius;
+        const int gb = gsblk(N, 256);
+        for (const int32_t p : hlabels) {
+            k_broad_x<R><<<gb, 256, 0, stream>>>(dims, p, kr, kw.as<R>(), phi.as<int32_t>(), work2.as<R>());
+            k_broad_yz<R, 1><<<gb, 256, 0, stream>>>(dims, kr, kw.as<R>(), work2.as<R>(), work3.as<R>());
+            k_broad_yz<R, 2><<<gb, 256, 0, stream>>>(dims, kr, kw.as<R>(), work3.as<R>(), work4.as<R>());
+            k_broad_fd<R><<<nblk(nb, 256), 256, 0, stream>>>(dims, p, nb, bandA.as<int>(), phi.as<int32_t>(),
+                                                            work4.as<R>(), dx_f, W.sigma_cells, nx.as<R>(),
+                                                            ny.as<R>(), nz.as<R>(), kappa.as<R>());
+        }
+        k_broad_finish<R><<<nblk(nb, 256), 256, 0, stream>>>(
+            dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(),
+            ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
+        HIPCHK(hipGetLastError());
+    }
+
     void launch_geometry(int nb, const GeomParams& gp) {
+        if (broad) {
+            broad_geometry(nb);
+            return;
+        }
         if (!kDirect && (dim3 ? try_tiled<3>(gp) : try_tiled<2>(gp))) return;
         if (dim3)
             k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
@@ -1191,14 +1252,6 @@ int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model, gpk_handl
         if (1LL * grid->Nz * grid->Ny * grid->Nx >= (1LL << 31))
             throw GpkError(GPK_E_INVALID, "grid exceeds 2^31 cells");
         if (!(grid->dx > 0.f)) throw GpkError(GPK_E_INVALID, "dx must be positive");
-        {
-            const float sc = std::max(grid->gaussian_sigma, 0.5f);
-            const int rg = static_cast<int>(std::ceil(3.5f * sc)) + 2;
-            if (grid->Nz > 1 && rg > 18)
-                throw GpkError(GPK_E_INVALID,
-                               "3-D geometry radius > 18 selects the reference's global separable "
-                               "path, which this library does not implement");
-        }
         if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_ANISO_OSTWALD)
             throw GpkError(GPK_E_UNSUPPORTED_MODEL, "unknown model kind");
         int ndev = 0;

@@ -322,7 +322,9 @@ struct Grid {
         law.d = m;
         gauss = make_gauss<R>(sigma);
         geometry_kernels(gauss, w0, w1, w2, rg);
-        lut = make_row_lut(w0, w1, w2);
+        // row tables of the separable modes (64-bit window rows); the broad
+        // 3-D path and the direct stencil do not use them
+        if (2 * rg + 1 <= 64) lut = make_row_lut(w0, w1, w2);
         // src/gpk_grid.cpp:24-27 (GaussKernel clamps sigma to >= 0.5)
         vel_sigma = std::max(Nz > 1 ? std::clamp(0.375f * sigma, 0.85f, 3.00f) : 0.85f, 0.5f);
         remap_sigma = std::max(Nz > 1 ? std::clamp(0.125f * sigma, 0.70f, 1.00f) : 0.70f, 0.5f);
@@ -389,7 +391,117 @@ struct Grid {
     }
 
     // -- src/gpk_geometry.cpp:88-183 -------------------------------------------
+    // -- src/gpk_geometry.cpp:185-333: 3-D grids whose geometry radius exceeds
+    // 18 (src/gpk_grid.cpp:44).  Per interface label, in ascending order: the
+    // 0/1 indicator smoothed by the plain Gaussian in x, then y, then z
+    // (k-ordered sums, separate multiply and add), fourth-order centred
+    // differences at the label's interface cells, normal from 1/|grad I|.
+    bool broad() const { return Nz > 1 && rg > 18; }
+    void broad_geometry() {
+        std::vector<int32_t> labs;
+        for (int i = 0; i < N; ++i) if (mask[i]) labs.push_back(phi[i]);
+        std::sort(labs.begin(), labs.end());
+        labs.erase(std::unique(labs.begin(), labs.end()), labs.end());
+        std::fill(nx.begin(), nx.end(), R(0));
+        std::fill(ny.begin(), ny.end(), R(0));
+        std::fill(nz.begin(), nz.end(), R(0));
+        std::fill(kappa.begin(), kappa.end(), R(0));
+        const int kr = gauss.radius;
+        const R idx12 = R(1) / (R(12) * dx);
+        const R idx_sq = R(1) / (dx * dx);
+        const R idx12_sq = idx_sq / R(12);
+        const R idx144_sq = idx_sq / R(144);
+        const R klim = R(1) / (R(gauss.sigma_cells) * dx);
+        std::vector<R> t1(N), t2(N), sm(N);
+        for (const int32_t p : labs) {
+            #pragma omp parallel for schedule(static)
+            for (int zy = 0; zy < Nz * Ny; ++zy) {
+                const int base = zy * Nx;
+                for (int x = 0; x < Nx; ++x) {
+                    R acc = R(0);
+                    for (int k = 0; k <= 2 * kr; ++k) {
+                        const int xx = wrap(x + k - kr, Nx);
+                        acc += gauss.w[k] * (phi[base + xx] == p ? R(1) : R(0));
+                    }
+                    t1[base + x] = acc;
+                }
+            }
+            #pragma omp parallel for schedule(static)
+            for (int zy = 0; zy < Nz * Ny; ++zy) {
+                const int z = zy / Ny, y = zy % Ny;
+                for (int x = 0; x < Nx; ++x) {
+                    R acc = R(0);
+                    for (int k = 0; k <= 2 * kr; ++k)
+                        acc += gauss.w[k] * t1[idx(z, wrap(y + k - kr, Ny), x)];
+                    t2[idx(z, y, x)] = acc;
+                }
+            }
+            #pragma omp parallel for schedule(static)
+            for (int zy = 0; zy < Nz * Ny; ++zy) {
+                const int z = zy / Ny, y = zy % Ny;
+                for (int x = 0; x < Nx; ++x) {
+                    R acc = R(0);
+                    for (int k = 0; k <= 2 * kr; ++k)
+                        acc += gauss.w[k] * t2[idx(wrap(z + k - kr, Nz), y, x)];
+                    sm[idx(z, y, x)] = acc;
+                }
+            }
+            const int na = static_cast<int>(active.size());
+            #pragma omp parallel for schedule(static)
+            for (int a = 0; a < na; ++a) {
+                const int i = active[a];
+                if (phi[i] != p) continue;
+                int z, y, x; decode(i, z, y, x);
+                const int xo[4] = {wrap(x - 2, Nx), wrap(x - 1, Nx), wrap(x + 1, Nx), wrap(x + 2, Nx)};
+                const int yo[4] = {wrap(y - 2, Ny), wrap(y - 1, Ny), wrap(y + 1, Ny), wrap(y + 2, Ny)};
+                const int zo[4] = {wrap(z - 2, Nz), wrap(z - 1, Nz), wrap(z + 1, Nz), wrap(z + 2, Nz)};
+                auto S = [&](int zz, int yy, int xx) { return sm[idx(zz, yy, xx)]; };
+                const R I0 = sm[i];
+                const R Ix = (S(z, y, xo[0]) - R(8) * S(z, y, xo[1]) + R(8) * S(z, y, xo[2]) - S(z, y, xo[3])) * idx12;
+                const R Iy = (S(z, yo[0], x) - R(8) * S(z, yo[1], x) + R(8) * S(z, yo[2], x) - S(z, yo[3], x)) * idx12;
+                const R Iz = (S(zo[0], y, x) - R(8) * S(zo[1], y, x) + R(8) * S(zo[2], y, x) - S(zo[3], y, x)) * idx12;
+                const R Ixx = (-S(z, y, xo[3]) + R(16) * S(z, y, xo[2]) - R(30) * I0 + R(16) * S(z, y, xo[1])
+                               - S(z, y, xo[0])) * idx12_sq;
+                const R Iyy = (-S(z, yo[3], x) + R(16) * S(z, yo[2], x) - R(30) * I0 + R(16) * S(z, yo[1], x)
+                               - S(z, yo[0], x)) * idx12_sq;
+                const R Izz = (-S(zo[3], y, x) + R(16) * S(zo[2], y, x) - R(30) * I0 + R(16) * S(zo[1], y, x)
+                               - S(zo[0], y, x)) * idx12_sq;
+                const R dc[4] = {R(1), R(-8), R(8), R(-1)};
+                R sxy = 0, sxz = 0, syz = 0;
+                for (int aa = 0; aa < 4; ++aa)
+                    for (int bb = 0; bb < 4; ++bb) {
+                        const R w = dc[aa] * dc[bb];
+                        sxy += w * S(z, yo[bb], xo[aa]);
+                        sxz += w * S(zo[bb], y, xo[aa]);
+                        syz += w * S(zo[bb], yo[aa], x);
+                    }
+                const R Ixy = sxy * idx144_sq, Ixz = sxz * idx144_sq, Iyz = syz * idx144_sq;
+                const R gsq = Ix * Ix + Iy * Iy + Iz * Iz;
+                if (gsq * dx * dx > R(K<R>::grad_eps)) {
+                    const R inv = R(1) / std::sqrt(gsq);
+                    nx[i] = -Ix * inv; ny[i] = -Iy * inv; nz[i] = -Iz * inv;
+                    const R trH = Ixx + Iyy + Izz;
+                    const R gHg = Ix * Ix * Ixx + Iy * Iy * Iyy + Iz * Iz * Izz
+                                + R(2) * (Ix * Iy * Ixy + Ix * Iz * Ixz + Iy * Iz * Iyz);
+                    R kap = (gHg - gsq * trH) / (gsq * std::sqrt(gsq));
+                    if (std::fabs(kap * dx) < R(K<R>::kappa_eps)) kap = R(0);
+                    kappa[i] = std::clamp(kap, -klim, klim);
+                }
+            }
+        }
+        const int na = static_cast<int>(active.size());
+        #pragma omp parallel for schedule(static)
+        for (int a = 0; a < na; ++a) {
+            const int i = active[a];
+            phi_comp[i] = competitor(i);
+        }
+    }
+
     void geometry() {
+        if (broad()) {
+            broad_geometry();
+            return;
+        }
         std::fill(nx.begin(), nx.end(), R(0));
         std::fill(ny.begin(), ny.end(), R(0));
         std::fill(nz.begin(), nz.end(), R(0));
@@ -433,6 +545,7 @@ struct Grid {
                 }
             } else {
                 // separable: masked row sums over x, FMA over y, FMA over z
+                if (lut.nch == 0) std::abort();   // window rows wider than 64 cells
                 const int dzb = Nz > 1 ? -r : 0, dze = Nz > 1 ? r : 0;
                 for (int dz = dzb; dz <= dze; ++dz) {
                     const int z = wrap(z0 + dz, Nz);

@@ -61,8 +61,6 @@ def test_errors_are_status_codes():
     bad = _abi.isotropic(1.0, 1.0)
     bad.kind = 17
     assert L.gpk_create(C.byref(gd), C.byref(bad), C.byref(h)) == _abi.GPK_E_UNSUPPORTED_MODEL
-    gd3 = _abi.GridDesc(16, 16, 16, 1.0, 0.0, 5.0, 0, 0)    # 3-D radius > 18: not built
-    assert L.gpk_create(C.byref(gd3), C.byref(m), C.byref(h)) == _abi.GPK_E_INVALID
     assert L.gpk_step(None, -1.0, 1e30, None, None) == _abi.GPK_E_INVALID
 
 

@@ -79,6 +79,15 @@ def test_grain_growth_sigma2_3d_strict32():
     lockstep(o, G, 24, check_every=3)
 
 
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_broad_kernel_geometry_3d(mode):
+    # sigma 5 cells: 3-D geometry radius 20 > 18, per-label separable
+    # smoothing + fourth-order differences (src/gpk_geometry.cpp:185-390)
+    phi, g = O.voronoi(32, 36, 40, 40, 3)
+    o, G = pair((32, 36, 40), _abi.isotropic(1.0, 1.0), 0.0, 5.0, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 20, check_every=4)
+
+
 def test_aniso_ostwald_lamella_2d_strict32():
     model = _abi.aniso_ostwald(1e-8, 0.5, 1.0, 4.0, 1.0, 0.1, 1e8)
     phi = np.zeros((96, 128), np.int32)

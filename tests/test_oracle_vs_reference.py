@@ -91,3 +91,10 @@ def test_topology_simple_move_matches_reference():
     for cell in rng.integers(0, p.size, 400):
         for nl in {int(p[(cell + 1) % p.size]), int(p[(cell - 48) % p.size]), int(p[cell]), 999}:
             assert o.simple_move(int(cell), nl) == r.simple_move(int(cell), nl)
+
+
+def test_broad_kernel_geometry_3d():
+    # sigma 5 cells: geometry radius 20 > 18 selects the per-label separable
+    # smoothing and fourth-order differences (src/gpk_geometry.cpp:185-390)
+    phi, g = O.voronoi(32, 36, 40, 40, 3)
+    run_pair((32, 36, 40), _abi.isotropic(1.0, 1.0), 0.0, 5.0, 20, {"phi": phi, "g": g})
