@@ -36,10 +36,13 @@ ANISO_GROWTH = 1
 SOLIDIFICATION = 2
 TENSOR_ANISO = 3
 ANISO_OSTWALD = 4
+RECRYSTALLISATION = 5
+DISLOCATION = 6
 
 FLAG_GRAIN_NETWORK = 0x1
 FLAG_SOLUTE_REJECTION = 0x2
 FLAG_STOCHASTIC = 0x4
+FLAG_DISLOCATION = 0x8
 
 
 class ModelDesc(C.Structure):
@@ -51,6 +54,8 @@ class ModelDesc(C.Structure):
         ("m_liq", C.c_float), ("k_part", C.c_float), ("T_melt", C.c_float), ("L", C.c_float),
         ("bxx", C.c_float), ("byy", C.c_float), ("bzz", C.c_float),
         ("c_sat", C.c_float), ("Omega", C.c_float), ("dG_bias", C.c_float),
+        ("E_stored", C.c_float), ("disloc_coeff", C.c_float), ("rho_rex", C.c_float),
+        ("recovery_rate", C.c_float),
     ]
 
 
@@ -142,6 +147,27 @@ def tensor_aniso(M0, sigma0, bxx=1.0, byy=1.0, bzz=1.0, dG_bias=0.0):
     m = model_defaults(TENSOR_ANISO)
     m.M0, m.sigma0 = M0, sigma0
     m.bxx, m.byy, m.bzz, m.dG_bias = bxx, byy, bzz, dG_bias
+    return m
+
+
+def recrystallisation(M0, sigma0, E_stored):
+    """RecrystallisationThermo(M0, sigma0, E_stored) (include/gpk_thermo.hpp:90-111):
+    label 0 is the deformed matrix, labels >= 1 recrystallised grains."""
+    m = model_defaults(RECRYSTALLISATION)
+    m.M0, m.sigma0, m.E_stored = M0, sigma0, E_stored
+    return m
+
+
+def dislocation(M0, sigma0, mu, b, alphaT=0.5, recovery_rate=0.0, rho_rex=0.0):
+    """DislocationThermo (include/gpk_thermo.hpp:135-170): stored energy
+    disloc_coeff * rho with disloc_coeff = 0.5 alphaT mu b^2 in float, a swept
+    boundary leaves rho_rex behind, static recovery at recovery_rate."""
+    f = lambda v: C.c_float(v).value  # noqa: E731
+    coeff = f(f(f(f(0.5 * f(alphaT)) * f(mu)) * f(b)) * f(b))
+    m = model_defaults(DISLOCATION)
+    m.M0, m.sigma0 = M0, sigma0
+    m.flags |= FLAG_DISLOCATION
+    m.disloc_coeff, m.recovery_rate, m.rho_rex = coeff, recovery_rate, rho_rex
     return m
 
 

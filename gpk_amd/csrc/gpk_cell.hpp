@@ -75,7 +75,8 @@ template <class R, int DIM, class CubeAt>
 __device__ __forceinline__ void comp_vel_cell(
     const Dims& d, const gpk_model_desc& model, int i, int a, int z0, int y0, int x0, int32_t res,
     R nxv, R nyv, R nzv, R kap, CubeAt cube_at, const int32_t* __restrict__ phi,
-    const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T, R* __restrict__ vraw,
+    const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+    const R* __restrict__ rho, R* __restrict__ vraw,
     int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     (void)cube_at;
 
@@ -122,9 +123,17 @@ __device__ __forceinline__ void comp_vel_cell(
         if (phi[f[k]] == qid) { gq = g[f[k]]; cq = c[f[k]]; break; }
     g_comp[i] = gq;
     const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
-    const R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
+    R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
     const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
     const R mob = R(model.M0);
+    // stored dislocation energy of the product side minus the parent's
+    // (src/gpk_physics.cpp:131-140)
+    if (model.flags & GPK_FLAG_DISLOCATION) {
+        R rp = rho[i];
+        for (int k = 0; k < 6; ++k)
+            if (phi[f[k]] == qid) { rp = rho[f[k]]; break; }
+        dG += R(model.disloc_coeff) * (rp - rho[i]);
+    }
     vraw[a] = mob * (dG - sig * kap);
 }
 
@@ -135,7 +144,8 @@ __device__ __forceinline__ void finish_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
     int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
     const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
-    const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
+    const R* __restrict__ T, const R* __restrict__ rho, R* __restrict__ onx, R* __restrict__ ony,
+    R* __restrict__ onz,
     R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
     R* __restrict__ g_comp) {
     const R dx = R(gp.dx_f);
@@ -159,7 +169,7 @@ __device__ __forceinline__ void finish_cell(
     }
     onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
     comp_vel_cell<R, DIM>(d, model, i, a, z0, y0, x0, res, nxv, nyv, nzv, kap, cube_at, phi, g, c, T,
-                          vraw, phi_comp, g_comp);
+                          rho, vraw, phi_comp, g_comp);
 }
 
 
@@ -171,6 +181,7 @@ __device__ __forceinline__ void geometry_cell(
     const int32_t* __restrict__ phi, const R* __restrict__ w0, const R* __restrict__ w1,
     const R* __restrict__ w2, const R* __restrict__ tab, const RowLutDesc& lutd,
     const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+    const R* __restrict__ rho,
     R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz, R* __restrict__ okappa,
     R* __restrict__ vraw, int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     const int32_t res = phi[i];
@@ -240,7 +251,7 @@ __device__ __forceinline__ void geometry_cell(
         return phi[lin(d, zz, yy, xx)];
     };
     finish_cell<R, DIM>(d, gp, model, i, a, z0, y0, x0, res, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz, Iyz,
-                        cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                        cube_at, phi, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
 

@@ -158,6 +158,10 @@ __device__ __forceinline__ R law_driving_force(const gpk_model_desc& d, int32_t 
     case GPK_MODEL_ANISO_OSTWALD:
         if (pp == 0) return R(d.Omega) * (R(d.c_sat) - cv);
         return R(d.Omega) * (cv - R(d.c_sat));
+    case GPK_MODEL_RECRYSTALLISATION:   // include/gpk_thermo.hpp:100-108
+        if (pp == 0 && pq >= 1) return -R(d.E_stored);
+        if (pp >= 1 && pq == 0) return R(d.E_stored);
+        return R(0);
     default:
         return R(0);
     }

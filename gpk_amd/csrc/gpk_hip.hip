@@ -606,7 +606,7 @@ struct Engine final : EngineBase {
         const int grid = 8 * ((ntiles + 7) / 8);
         k_geometry_tiled<R, DIM, RG><<<grid, 256, 0, stream>>>(
             dims, gp, model, ntx, nty, ntz, phi.as<int32_t>(), mask.as<uint8_t>(), slot.as<int>(),
-            wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(),
+            wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(),
             nx.as<R>(), ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(),
             g_comp.as<R>());
     }
@@ -652,7 +652,7 @@ struct Engine final : EngineBase {
                                                             ny.as<R>(), nz.as<R>(), kappa.as<R>());
         }
         k_broad_finish<R><<<nblk(nb, 256), 256, 0, stream>>>(
-            dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(),
+            dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(),
             ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
         HIPCHK(hipGetLastError());
     }
@@ -666,12 +666,12 @@ struct Engine final : EngineBase {
         if (dim3)
             k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                 dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(), ny.as<R>(),
                 nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
         else
             k_geometry<R, 2, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                 dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), nx.as<R>(), ny.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(), ny.as<R>(),
                 nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
     }
 
@@ -705,6 +705,8 @@ struct Engine final : EngineBase {
         pull_scalars();
         return hs->nsel;
     }
+
+    int sweep_rho() const { return (model.flags & GPK_FLAG_DISLOCATION) ? 1 : 0; }
 
     // pair projection of vraw into vn (src/gpk_physics.cpp:152-193), vmax on the side
     void project(int nb) {
@@ -852,6 +854,17 @@ struct Engine final : EngineBase {
                                                               alpha.as<R>(), ds(), dx_f, D_f, work2.as<R>());
             c.swap(work2);
         }
+        // static recovery of rho (src/gpk_transport.cpp:142-152); the decay
+        // factor uses the host libm, as the reference does
+        if ((model.flags & GPK_FLAG_DISLOCATION) && model.recovery_rate > 0.f) {
+            pull_scalars();
+            const R dtr = R(hs->dt);
+            if (dtr > R(0)) {
+                const R decay = sizeof(R) == 4 ? R(std::exp(-model.recovery_rate * float(dtr)))
+                                               : R(std::exp(-double(model.recovery_rate) * double(dtr)));
+                k_scale_r<R><<<gsblk(N, 256), 256, 0, stream>>>(N, rho.as<R>(), decay);
+            }
+        }
         k_time_advance<<<1, 1, 0, stream>>>(ds());
         ++step_id;
         if (st) {
@@ -931,7 +944,7 @@ struct Engine final : EngineBase {
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
             phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
-            network ? 0 : 1);
+            network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho());
         return m;
     }
 
@@ -1016,7 +1029,8 @@ struct Engine final : EngineBase {
                 }
                 k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
                     npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
-                    Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds());
+                    Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds(),
+                    rho.as<R>(), R(model.rho_rex), sweep_rho());
             }
         }
         band_compact(bandB, false);
@@ -1041,7 +1055,8 @@ struct Engine final : EngineBase {
         if (n > 0)
             k_erode<R><<<1, 1024, 0, stream>>>(dims, dim3, n, ulist.as<int>(), unres.as<uint8_t>(),
                                                phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(),
-                                               utgt.as<int>(), utgg.as<R>(), ds());
+                                               utgt.as<int>(), utgg.as<R>(), ds(), rho.as<R>(),
+                                               R(model.rho_rex), sweep_rho());
         band_compact(bandB, false);
     }
 
@@ -1252,7 +1267,7 @@ int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model, gpk_handl
         if (1LL * grid->Nz * grid->Ny * grid->Nx >= (1LL << 31))
             throw GpkError(GPK_E_INVALID, "grid exceeds 2^31 cells");
         if (!(grid->dx > 0.f)) throw GpkError(GPK_E_INVALID, "dx must be positive");
-        if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_ANISO_OSTWALD)
+        if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_DISLOCATION)
             throw GpkError(GPK_E_UNSUPPORTED_MODEL, "unknown model kind");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)

@@ -48,12 +48,13 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
            const R* __restrict__ w0, const R* __restrict__ w1, const R* __restrict__ w2,
            const R* __restrict__ tab, RowLutDesc lutd,
            const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+           const R* __restrict__ rho,
            R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
            R* __restrict__ okappa, R* __restrict__ vraw,
            int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nband) return;
-    geometry_cell<R, DIM, kDirect>(d, gp, model, a, band[a], phi, w0, w1, w2, tab, lutd, g, c, T, onx,
+    geometry_cell<R, DIM, kDirect>(d, gp, model, a, band[a], phi, w0, w1, w2, tab, lutd, g, c, T, rho, onx,
                                    ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
@@ -105,7 +106,8 @@ __device__ __forceinline__ void tiled_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0,
     int32_t p, const R* lut, const R (*wy)[2 * RG + 1], RowMask rowmask,
     const int32_t* __restrict__ phi, const int* __restrict__ slot, const R* __restrict__ g,
-    const R* __restrict__ c, const R* __restrict__ T, R* __restrict__ onx, R* __restrict__ ony,
+    const R* __restrict__ c, const R* __restrict__ T, const R* __restrict__ rho,
+    R* __restrict__ onx, R* __restrict__ ony,
     R* __restrict__ onz, R* __restrict__ okappa, R* __restrict__ vraw,
     int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
@@ -150,7 +152,7 @@ __device__ __forceinline__ void tiled_cell(
         return phi[lin(d, zz, yy, xx)];
     };
     finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, p, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz,
-                        Iyz, cube_at, phi, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                        Iyz, cube_at, phi, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
 template <class R, int DIM, int RG>
@@ -161,6 +163,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  const R* __restrict__ w1, const R* __restrict__ w2,
                  const R* __restrict__ lut_g,
                  const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
+                 const R* __restrict__ rho,
                  R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
                  R* __restrict__ okappa, R* __restrict__ vraw,
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
@@ -275,7 +278,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                                    [&](int jz, int jy) {
                                        return rows[(jz * S::SY + jy) * GEO_MAXRES] >> lx;
                                    },
-                                   phi, slot, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                                   phi, slot, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
         }
     }
     // residents that did not fit the hash: window rows read from the grid
@@ -296,7 +299,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                                            m |= 1ull << jx;
                                    return m;
                                },
-                               phi, slot, g, c, T, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                               phi, slot, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
     }
 }
 
@@ -1087,7 +1090,7 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
                              const R* __restrict__ Canew, const int* __restrict__ state,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
-                             int all_accept) {
+                             int all_accept, R* __restrict__ rho, R rho_rex, int sweep) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool acc = false, def = false;
     if (k < m) {
@@ -1097,6 +1100,7 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
             phi[cell] = Cnew[k];
             g[cell] = Cg[k];
             alpha[cell] = R(1);
+            if (sweep) rho[cell] = rho_rex;   // src/gpk_transitions.cpp:308
             atomicSub(count + Cold[k], 1);
             atomicAdd(count + Cnew[k], 1);
             acc = true;
@@ -1367,7 +1371,8 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
                              const int* __restrict__ Plo, const int* __restrict__ Phi,
                              const int* __restrict__ pstate, const R* __restrict__ Pg,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
-                             int* __restrict__ pid, Scalars* s) {
+                             int* __restrict__ pid, Scalars* s, R* __restrict__ rho, R rho_rex,
+                             int sweep) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const int a = Pa[q], b = Pb[q];
@@ -1376,6 +1381,7 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
     phi[a] = Phi[q]; phi[b] = Plo[q];
     g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
     alpha[a] = R(1); alpha[b] = R(1);
+    if (sweep) { rho[a] = rho_rex; rho[b] = rho_rex; }   // src/gpk_transitions.cpp:173
     atomicAdd(&s->exchanges, 2);
 }
 
@@ -1404,7 +1410,8 @@ template <class R>
 __global__ void __launch_bounds__(1024)
 k_erode(Dims d, int dim3, int n, const int* __restrict__ cells, const uint8_t* __restrict__ unres,
         int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
-        int* __restrict__ count, int* __restrict__ tgt, R* __restrict__ tgg, Scalars* s) {
+        int* __restrict__ count, int* __restrict__ tgt, R* __restrict__ tgg, Scalars* s,
+        R* __restrict__ rho, R rho_rex, int sweep) {
     __shared__ int changed;
     const int nface = dim3 ? 6 : 4;
     while (true) {
@@ -1446,6 +1453,7 @@ k_erode(Dims d, int dim3, int n, const int* __restrict__ cells, const uint8_t* _
             phi[i] = tgt[e];
             g[i] = tgg[e];
             alpha[i] = R(1);
+            if (sweep) rho[i] = rho_rex;   // src/gpk_transitions.cpp:245
             atomicAdd(&changed, 1);
         }
         __syncthreads();
@@ -1647,6 +1655,11 @@ __global__ void k_fill_u8(int n, uint8_t* p, uint8_t v) {
 template <class R>
 __global__ void k_fill_r(int n, R* p, R v) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <class R>
+__global__ void k_scale_r(int n, R* __restrict__ p, R f) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] *= f;
 }
 
 __global__ void k_minmax_i32(int N, const int32_t* __restrict__ p, int* __restrict__ out) {

@@ -78,13 +78,16 @@ enum gpk_model_kind {
     GPK_MODEL_ANISO_GROWTH   = 1,  /* AnisotropicGrowthThermo include/gpk_thermo.hpp:33-45  */
     GPK_MODEL_SOLIDIFICATION = 2,  /* SolidificationThermo    include/gpk_thermo.hpp:52-85  */
     GPK_MODEL_TENSOR_ANISO   = 3,  /* TensorAnisotropyThermo  include/gpk_thermo.hpp:229-263 */
-    GPK_MODEL_ANISO_OSTWALD  = 4   /* AnisoOstwaldThermo      include/gpk_thermo.hpp:284-325 */
+    GPK_MODEL_ANISO_OSTWALD  = 4,  /* AnisoOstwaldThermo      include/gpk_thermo.hpp:284-325 */
+    GPK_MODEL_RECRYSTALLISATION = 5, /* RecrystallisationThermo include/gpk_thermo.hpp:90-111 */
+    GPK_MODEL_DISLOCATION    = 6   /* DislocationThermo       include/gpk_thermo.hpp:135-170 */
 };
 
 /* flag bits of gpk_model_desc.flags (ThermoModel fields, include/gpk_core.hpp:72-123) */
 #define GPK_FLAG_GRAIN_NETWORK    0x1u
 #define GPK_FLAG_SOLUTE_REJECTION 0x2u
 #define GPK_FLAG_STOCHASTIC       0x4u
+#define GPK_FLAG_DISLOCATION      0x8u   /* dislocation_enabled: rho coupling, sweep, recovery */
 
 typedef struct gpk_model_desc {
     int32_t  kind;          /* gpk_model_kind */
@@ -102,6 +105,11 @@ typedef struct gpk_model_desc {
     float bxx, byy, bzz;
     float c_sat, Omega;     /* AnisoOstwaldThermo */
     float dG_bias;          /* TensorAnisotropyThermo (mutable between steps) */
+    float E_stored;         /* RecrystallisationThermo */
+    /* stored-energy coupling (include/gpk_core.hpp:101-110), GPK_FLAG_DISLOCATION */
+    float disloc_coeff;     /* E_rho = disloc_coeff * rho */
+    float rho_rex;          /* density left behind a swept boundary */
+    float recovery_rate;    /* d(rho)/dt = -recovery_rate * rho */
 } gpk_model_desc;
 
 /* Fills *m with the defaults of the reference ThermoModel base and the given

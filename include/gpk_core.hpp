@@ -107,12 +107,16 @@ struct ThermoModel {
         d->kB = kB;
         d->flags = (grain_network ? GPK_FLAG_GRAIN_NETWORK : 0u) |
                    (solute_rejection ? GPK_FLAG_SOLUTE_REJECTION : 0u) |
-                   (stochastic ? GPK_FLAG_STOCHASTIC : 0u);
+                   (stochastic ? GPK_FLAG_STOCHASTIC : 0u) |
+                   (dislocation_enabled ? GPK_FLAG_DISLOCATION : 0u);
+        d->disloc_coeff = disloc_coeff;
+        d->rho_rex = rho_rex;
+        d->recovery_rate = recovery_rate;
     }
     // couplings that exist in the reference but not on the device path
+    // (the stored-energy coupling is on it: GPK_FLAG_DISLOCATION)
     bool couplings_off() const {
-        return !phase_selection && !elastic_enabled && !dislocation_enabled && !magnetic_enabled &&
-               Q == 0.f;
+        return !phase_selection && !elastic_enabled && !magnetic_enabled && Q == 0.f;
     }
 };
 

@@ -105,4 +105,48 @@ struct AnisoOstwaldThermo : ThermoModel {
     }
 };
 
+// include/gpk_thermo.hpp:90-111 (GPK source tree): label 0 is the deformed
+// matrix, labels >= 1 recrystallised grains; E_stored drives 0 -> rex.
+struct RecrystallisationThermo : ThermoModel {
+    float E_stored;
+    RecrystallisationThermo(float M0_val, float sigma0_v, float E_stored_v) : E_stored(E_stored_v) {
+        M0 = M0_val;
+        sigma0 = sigma0_v;
+        eps4 = 0.f;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_RECRYSTALLISATION);
+        d->E_stored = E_stored;
+        return true;
+    }
+};
+
+// include/gpk_thermo.hpp:135-170: stored energy disloc_coeff * rho with
+// disloc_coeff = 0.5 alphaT mu b^2, sweeping to rho_rex, static recovery.
+struct DislocationThermo : ThermoModel {
+    float mu, b, alphaT;
+    DislocationThermo(float M0_val, float sigma0_val, float mu_val, float b_val, float alphaT_val = 0.5f,
+                      float recovery_rate_val = 0.f, float rho_rex_val = 0.f)
+        : mu(mu_val), b(b_val), alphaT(alphaT_val) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = 0.f;
+        dislocation_enabled = true;
+        disloc_coeff = 0.5f * alphaT * mu * b * b;
+        recovery_rate = recovery_rate_val;
+        rho_rex = rho_rex_val;
+    }
+    float stored_energy(float rho_val) const { return disloc_coeff * rho_val; }
+    float critical_radius(float rho_val) const {
+        const float E = disloc_coeff * rho_val;
+        return (E > 0.f) ? sigma0 / E : 1e30f;
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (!couplings_off()) return false;
+        fill_common(d, GPK_MODEL_DISLOCATION);
+        return true;
+    }
+};
+
 } // namespace gpk

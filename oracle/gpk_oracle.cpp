@@ -222,6 +222,10 @@ struct Law {
         case GPK_MODEL_ANISO_OSTWALD:
             if (pp == 0) return R(d.Omega) * (R(d.c_sat) - cv);
             return R(d.Omega) * (cv - R(d.c_sat));
+        case GPK_MODEL_RECRYSTALLISATION:   // include/gpk_thermo.hpp:100-108
+            if (pp == 0 && pq >= 1) return -R(d.E_stored);
+            if (pp >= 1 && pq == 0) return R(d.E_stored);
+            return R(0);
         default:
             return R(0);
         }
@@ -647,9 +651,16 @@ struct Grid {
                 if (phi[f[k]] == q) { gq = g[f[k]]; cq = c[f[k]]; break; }
             g_comp[i] = gq;
             const R ci = law.interface_composition(phi[i], q, c[i], cq);
-            const R dG = law.driving_force(phi[i], q, ci, T[i]);
+            R dG = law.driving_force(phi[i], q, ci, T[i]);
             const R sig = law.energy(nx[i], ny[i], nz[i], gp);
             const R mob = R(law.d.M0);
+            // src/gpk_physics.cpp:131-140
+            if (law.d.flags & GPK_FLAG_DISLOCATION) {
+                R rp = rho[i];
+                for (int k = 0; k < 6; ++k)
+                    if (phi[f[k]] == q) { rp = rho[f[k]]; break; }
+                dG += R(law.d.disloc_coeff) * (rp - rho[i]);
+            }
             vn[i] = mob * (dG - sig * kappa[i]);
         }
     }
@@ -951,6 +962,7 @@ struct Grid {
                     continue;
                 }
                 phi[b] = lo; g[a] = ga; g[b] = gb; alpha[a] = alpha[b] = R(1);
+                if (law.d.flags & GPK_FLAG_DISLOCATION) rho[a] = rho[b] = R(law.d.rho_rex);
                 exchanges += 2;
             }
         }
@@ -995,6 +1007,7 @@ struct Grid {
             if (cells.empty()) break;
             for (size_t k = 0; k < cells.size(); ++k) {
                 phi[cells[k]] = tg[k]; g[cells[k]] = tgg[k]; alpha[cells[k]] = R(1);
+                if (law.d.flags & GPK_FLAG_DISLOCATION) rho[cells[k]] = R(law.d.rho_rex);
             }
             eroded += static_cast<int64_t>(cells.size());
         }
@@ -1044,6 +1057,7 @@ struct Grid {
             bool glob = false;
             if (!network || simple_move(i, nl, counts[ol], counts[nl], false, &glob, &st.bfs_visits)) {
                 phi[i] = nl; g[i] = g_comp[i]; alpha[i] = R(1);
+                if (law.d.flags & GPK_FLAG_DISLOCATION) rho[i] = R(law.d.rho_rex);
                 --counts[ol]; ++counts[nl];
                 ++st.accepted;
             } else {
@@ -1158,9 +1172,18 @@ struct Grid {
         alpha_and_flips(dt);
         if (part) { solute_rejection(); diffusion_partitioned(dt); }
         else diffusion(dt);
+        recovery(dt);
         time += double(dt);
         ++step_id;
         return dt;
+    }
+
+    // -- src/gpk_transport.cpp:142-152 ---------------------------------------------
+    void recovery(R dt) {
+        if (!(law.d.flags & GPK_FLAG_DISLOCATION) || law.d.recovery_rate <= 0.f || dt <= R(0)) return;
+        const R decay = sizeof(R) == 4 ? R(std::exp(-law.d.recovery_rate * float(dt)))
+                                       : R(std::exp(-double(law.d.recovery_rate) * double(dt)));
+        for (int i = 0; i < N; ++i) rho[i] *= decay;
     }
 
     // -- diagnostics (src/gpk_simulation.cpp:78-123) --------------------------------------

@@ -41,6 +41,11 @@ std::shared_ptr<gpk::ThermoModel> make_model(const gpk_model_desc& m) {
     case GPK_MODEL_ANISO_OSTWALD:
         t = std::make_shared<gpk::AnisoOstwaldThermo>(m.M0, m.sigma0, m.bxx, m.byy, m.bzz,
                                                        m.c_sat, m.Omega); break;
+    case GPK_MODEL_RECRYSTALLISATION:
+        t = std::make_shared<gpk::RecrystallisationThermo>(m.M0, m.sigma0, m.E_stored); break;
+    case GPK_MODEL_DISLOCATION:
+        // constructed with unit mu, b; the coefficient is taken from the descriptor
+        t = std::make_shared<gpk::DislocationThermo>(m.M0, m.sigma0, 1.f, 1.f); break;
     default:
         return nullptr;
     }
@@ -51,6 +56,10 @@ std::shared_ptr<gpk::ThermoModel> make_model(const gpk_model_desc& m) {
     t->k_partition = m.k_partition;
     t->noise_scale = m.noise_scale;
     t->kB = m.kB;
+    t->dislocation_enabled = (m.flags & GPK_FLAG_DISLOCATION) != 0;
+    t->disloc_coeff = m.disloc_coeff;
+    t->rho_rex = m.rho_rex;
+    t->recovery_rate = m.recovery_rate;
     return t;
 }
 

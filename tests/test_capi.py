@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     # sizes of the POD structs crossing the ABI
-    assert C.sizeof(_abi.ModelDesc) == 2 * 4 + 17 * 4
+    assert C.sizeof(_abi.ModelDesc) == 2 * 4 + 21 * 4
     assert C.sizeof(_abi.GridDesc) == 8 * 4
     assert C.sizeof(_abi.StepStats) == 6 * 8 + 2 * 8 + 8
 

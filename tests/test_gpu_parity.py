@@ -12,7 +12,7 @@ import pytest
 import gpk_amd
 from gpk_amd import _abi
 from oracle import oracle as O
-from tests.util import FIELDS_ALL, solidification_setup
+from tests.util import FIELDS_ALL, dislocation_rho, rex_setup, solidification_setup
 
 pytestmark = pytest.mark.gpu
 
@@ -86,6 +86,20 @@ def test_broad_kernel_geometry_3d(mode):
     phi, g = O.voronoi(32, 36, 40, 40, 3)
     o, G = pair((32, 36, 40), _abi.isotropic(1.0, 1.0), 0.0, 5.0, mode, {"phi": phi, "g": g})
     lockstep(o, G, 20, check_every=4)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp64"])
+def test_recrystallisation_2d(mode):
+    o, G = pair((1, 80, 80), _abi.recrystallisation(1e-9, 0.5, 2e6), 0.0, 1.5, mode, {"phi": rex_setup()},
+                dx=1e-6)
+    lockstep(o, G, 60, check_every=5)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_dislocation_sweep_and_recovery_2d(mode):
+    model = _abi.dislocation(1e-9, 0.5, 4e10, 2.5e-10, 0.5, recovery_rate=50.0, rho_rex=1e12)
+    o, G = pair((1, 80, 80), model, 0.0, 1.5, mode, {"phi": rex_setup(), "rho": dislocation_rho()}, dx=1e-6)
+    lockstep(o, G, 60, fields=FIELDS_ALL + ("rho",), check_every=5)
 
 
 def test_aniso_ostwald_lamella_2d_strict32():

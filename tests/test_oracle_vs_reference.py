@@ -7,7 +7,7 @@ import pytest
 
 from gpk_amd import _abi
 from oracle import oracle as O
-from tests.util import solidification_setup
+from tests.util import dislocation_rho, rex_setup, solidification_setup
 
 pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 
@@ -98,3 +98,32 @@ def test_broad_kernel_geometry_3d():
     # smoothing and fourth-order differences (src/gpk_geometry.cpp:185-390)
     phi, g = O.voronoi(32, 36, 40, 40, 3)
     run_pair((32, 36, 40), _abi.isotropic(1.0, 1.0), 0.0, 5.0, 20, {"phi": phi, "g": g})
+
+
+def test_recrystallisation_2d():
+    # RecrystallisationThermo: E_stored drives the deformed matrix (label 0)
+    # into the recrystallised discs
+    phi = rex_setup()
+    run_pair((1, 80, 80), _abi.recrystallisation(1e-9, 0.5, 2e6), 0.0, 1.5, 60, {"phi": phi}, dx=1e-6)
+
+
+def test_dislocation_sweep_and_recovery_2d():
+    # DislocationThermo: rho coupling in the driving force, swept cells take
+    # rho_rex, static recovery rho *= exp(-r dt) every step
+    model = _abi.dislocation(1e-9, 0.5, 4e10, 2.5e-10, 0.5, recovery_rate=50.0, rho_rex=1e12)
+    nz, ny, nx = 1, 80, 80
+    init = {"phi": rex_setup(), "rho": dislocation_rho()}
+    o = O.OracleGrid(nz, ny, nx, 1e-6, model, 0.0, 1.5, "strict32")
+    r = O.RefGrid(nz, ny, nx, 1e-6, model, 0.0, 1.5)
+    for name, arr in init.items():
+        o.set(name, arr)
+        r.set(name, arr)
+    accepted = 0
+    for s in range(60):
+        assert o.step() == r.step(), s
+        accepted += o.stats()["accepted"]
+        for f in FIELDS + ("rho",):
+            assert np.array_equal(o.get(f), r.get(f)), (f, s)
+    assert accepted > 0
+    rho = o.get("rho")
+    assert rho.min() < 1e12 and rho.max() < 1.5e14   # swept and recovered

@@ -69,3 +69,18 @@ def solidification_setup(N, rng_seed=7, c0=0.02, k=0.14, undercooling=0.5, r=6):
     dT = undercooling * abs(m_liq) * c0 * (1 - k) / k
     T = np.full(phi.shape, T_melt + m_liq * c0 - dT, np.float32)
     return model, phi, conc, T
+
+
+def rex_setup(N=80):
+    """Deformed matrix (label 0) with three recrystallised discs (labels 1-3)."""
+    y, x = np.mgrid[0:N, 0:N]
+    phi = np.zeros((N, N), np.int32)
+    for k, (cy, cx, r) in enumerate([(20, 20, 7), (55, 30, 9), (40, 62, 6)]):
+        phi[(y - cy) ** 2 + (x - cx) ** 2 <= r * r] = k + 1
+    return phi
+
+
+def dislocation_rho(N=80):
+    """Dislocation density varying along x, 1e14 m^-2 +- 50 %."""
+    x = np.arange(N)[None, :].repeat(N, 0)
+    return (1e14 * (1 + 0.5 * np.sin(x / 7.0))).astype(np.float32)
