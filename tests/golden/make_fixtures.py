@@ -20,8 +20,8 @@ Fixtures (inputs and expected outputs only):
                      run from oracle/_ref/gpk
   cli_*.npz          a deck (text, output dir templated as {out}) and every
                      snapshot file the reference CLI wrote for it (CSV, VTK and
-                     PPM bytes), demos/gpk.cpp run_grain_growth (195-282) and
-                     run_solidification (288-391)
+                     PPM bytes), demos/gpk.cpp run_grain_growth (195-282),
+                     run_solidification (288-391), run_recrystallisation (477-555)
 """
 import argparse
 import os
@@ -192,6 +192,32 @@ seed_radius = 4
 format = all
 dir = {out}
 prefix = s3
+""",
+    "cli_recrystallisation_2d": """[run]
+model = recrystallisation
+[grid]
+Nz = 1
+Ny = 64
+Nx = 64
+dx = 1.0e-6
+[time]
+t_end = 30.0
+n_snaps = 3
+[thermo]
+M0 = 1.0e-12
+sigma0 = 0.3
+E_stored = 4.0e5
+[nucleation]
+n_nuclei = 5
+nucleus_r = 3
+Jdot = 5.0e14
+seed = 42
+[geometry]
+sigma_cells = 1.5
+[output]
+format = all
+dir = {out}
+write_initial = true
 """,
     "cli_grain_growth_3d": """[run]
 model = grain_growth

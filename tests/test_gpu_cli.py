@@ -19,7 +19,7 @@ from tests.util import GOLDEN, REPO
 pytestmark = pytest.mark.gpu
 
 CLI = os.path.join(REPO, "gpk_amd", "gpk")
-EXACT = ("cli_grain_growth_2d", "cli_grain_growth_3d", "cli_solidification_3d")
+EXACT = ("cli_grain_growth_2d", "cli_grain_growth_3d", "cli_solidification_3d", "cli_recrystallisation_2d")
 
 
 def first_difference(a, b):
