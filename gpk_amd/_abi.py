@@ -27,7 +27,7 @@ FIELDS = {
     "phi": (0, "i32"), "alpha": (1, "real"), "c": (2, "real"), "T": (3, "real"),
     "g": (4, "real"), "rho": (5, "real"), "nx": (6, "real"), "ny": (7, "real"),
     "nz": (8, "real"), "kappa": (9, "real"), "vn": (10, "real"),
-    "phi_comp": (11, "i32"), "g_comp": (12, "real"), "mask": (13, "u8"),
+    "phi_comp": (11, "i32"), "g_comp": (12, "real"), "mask": (13, "u8"), "E_el": (14, "real"),
 }
 
 # model kinds
@@ -38,11 +38,14 @@ TENSOR_ANISO = 3
 ANISO_OSTWALD = 4
 RECRYSTALLISATION = 5
 DISLOCATION = 6
+PRECIPITATION = 7
+MAX_PHASES = 8
 
 FLAG_GRAIN_NETWORK = 0x1
 FLAG_SOLUTE_REJECTION = 0x2
 FLAG_STOCHASTIC = 0x4
 FLAG_DISLOCATION = 0x8
+FLAG_ELASTIC = 0x10
 
 
 class ModelDesc(C.Structure):
@@ -56,6 +59,10 @@ class ModelDesc(C.Structure):
         ("c_sat", C.c_float), ("Omega", C.c_float), ("dG_bias", C.c_float),
         ("E_stored", C.c_float), ("disloc_coeff", C.c_float), ("rho_rex", C.c_float),
         ("recovery_rate", C.c_float),
+        ("elastic_modulus", C.c_float), ("eigenstrain_coeff", C.c_float), ("c_ref", C.c_float),
+        ("elastic_smooth_iters", C.c_int32),
+        ("n_phases", C.c_int32), ("T_ref", C.c_float),
+        ("pa", C.c_float * 8), ("pb", C.c_float * 8), ("pc2", C.c_float * 8), ("pdT", C.c_float * 8),
     ]
 
 
@@ -98,6 +105,8 @@ def model_defaults(kind):
     m.m_liq, m.k_part, m.T_melt, m.L = -500.0, 0.3, 933.0, 1e8
     m.bxx = m.byy = m.bzz = 1.0
     m.c_sat, m.Omega, m.dG_bias = 0.0, 0.0, 0.0
+    m.elastic_smooth_iters = 8
+    m.T_ref = 900.0
     if kind in (ISOTROPIC, ANISO_GROWTH):
         m.flags = FLAG_GRAIN_NETWORK
     if kind == SOLIDIFICATION:
@@ -168,6 +177,29 @@ def dislocation(M0, sigma0, mu, b, alphaT=0.5, recovery_rate=0.0, rho_rex=0.0):
     m.M0, m.sigma0 = M0, sigma0
     m.flags |= FLAG_DISLOCATION
     m.disloc_coeff, m.recovery_rate, m.rho_rex = coeff, recovery_rate, rho_rex
+    return m
+
+
+def precipitation(M0, sigma0, a, b, c2, dT, T_ref=900.0):
+    """PrecipitationThermo (include/gpk_thermo.hpp:333-387): phases 0..n-1 with
+    G_q = a_q + b_q c + c2_q c^2 + dT_q (T - T_ref); the product phase of a
+    cell is its lowest-G unlike face label."""
+    n = len(a)
+    if not (len(b) == len(c2) == len(dT) == n) or n > MAX_PHASES:
+        raise ValueError("precipitation: 1..%d phases with four coefficient lists" % MAX_PHASES)
+    m = model_defaults(PRECIPITATION)
+    m.M0, m.sigma0, m.n_phases, m.T_ref = M0, sigma0, n, T_ref
+    for q in range(n):
+        m.pa[q], m.pb[q], m.pc2[q], m.pdT[q] = a[q], b[q], c2[q], dT[q]
+    return m
+
+
+def with_elastic(m, modulus, eigenstrain_coeff, c_ref, smooth_iters=8):
+    """Switch on the elastic coupling of any model (ThermoModel::elastic_enabled,
+    src/gpk_physics.cpp:10-63)."""
+    m.flags |= FLAG_ELASTIC
+    m.elastic_modulus, m.eigenstrain_coeff, m.c_ref = modulus, eigenstrain_coeff, c_ref
+    m.elastic_smooth_iters = smooth_iters
     return m
 
 

@@ -217,7 +217,7 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
     // geometry, competitor, constitutive velocity
     for (int a = tid; a < nb; a += WG)
         geometry_cell<R, 2, kDirect>(d, A.gp, A.model, a, band[a], phi, A.w0, A.w1, A.w2, A.tab, A.lutd, g,
-                                     c, T, static_cast<const R*>(nullptr), bnx, bny, bnz, bk, vraw, phi_comp, g_comp);
+                                     c, T, static_cast<const R*>(nullptr), static_cast<const R*>(nullptr), bnx, bny, bnz, bk, vraw, phi_comp, g_comp);
     __syncthreads();
     // pair projection and the interface speed bound
     R vm = R(Eps<R>::vmax0);
@@ -724,7 +724,8 @@ int gpk_batch_create(int n_inst, const int* Ny, const int* Nx, float dx, float D
             if (Ny[b] < 3 || Nx[b] < 3) throw GpkError(GPK_E_INVALID, "instances need Ny, Nx >= 3");
         if (D != 0.f) throw GpkError(GPK_E_UNSUPPORTED_MODEL, "batched instances run with D = 0");
         if (model->flags & (GPK_FLAG_GRAIN_NETWORK | GPK_FLAG_SOLUTE_REJECTION | GPK_FLAG_STOCHASTIC |
-                            GPK_FLAG_DISLOCATION))
+                            GPK_FLAG_DISLOCATION | GPK_FLAG_ELASTIC) ||
+            model->kind == GPK_MODEL_PRECIPITATION)
             throw GpkError(GPK_E_UNSUPPORTED_MODEL,
                            "batched instances: models without grain-network topology, solute rejection "
                            "or noise");

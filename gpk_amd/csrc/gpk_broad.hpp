@@ -117,7 +117,8 @@ template <class R>
 __global__ void __launch_bounds__(256)
 k_broad_finish(Dims d, gpk_model_desc model, int nb, const int* __restrict__ band,
                const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
-               const R* __restrict__ T, const R* __restrict__ rho, const R* __restrict__ onx, const R* __restrict__ ony,
+               const R* __restrict__ T, const R* __restrict__ rho, const R* __restrict__ E_el,
+               const R* __restrict__ onx, const R* __restrict__ ony,
                const R* __restrict__ onz, const R* __restrict__ okappa, R* __restrict__ vraw,
                int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,7 +131,7 @@ k_broad_finish(Dims d, gpk_model_desc model, int nb, const int* __restrict__ ban
         return phi[lin(d, zz, yy, xx)];
     };
     comp_vel_cell<R, 3>(d, model, i, a, z0, y0, x0, phi[i], onx[a], ony[a], onz[a], okappa[a], cube_at,
-                        phi, g, c, T, rho, vraw, phi_comp, g_comp);
+                        phi, g, c, T, rho, E_el, vraw, phi_comp, g_comp);
 }
 
 } // namespace gpkd

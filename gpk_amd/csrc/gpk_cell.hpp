@@ -76,7 +76,7 @@ __device__ __forceinline__ void comp_vel_cell(
     const Dims& d, const gpk_model_desc& model, int i, int a, int z0, int y0, int x0, int32_t res,
     R nxv, R nyv, R nzv, R kap, CubeAt cube_at, const int32_t* __restrict__ phi,
     const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
-    const R* __restrict__ rho, R* __restrict__ vraw,
+    const R* __restrict__ rho, const R* __restrict__ E_el, R* __restrict__ vraw,
     int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     (void)cube_at;
 
@@ -113,7 +113,22 @@ __device__ __forceinline__ void comp_vel_cell(
             best = k; best_support = support; best_score = score[k];
         }
     }
-    const int32_t qid = best >= 0 ? phi[f[best]] : res;
+    int32_t qid = best >= 0 ? phi[f[best]] : res;
+    // phase selection (src/gpk_physics.cpp:79-95): the lowest-G unlike face
+    // label replaces the geometric competitor
+    if (model.kind == GPK_MODEL_PRECIPITATION) {
+        int32_t cand[6];
+        int nc = 0;
+        for (int k = 0; k < 6; ++k) {
+            const int32_t q = phi[f[k]];
+            if (q == res) continue;
+            bool seen = false;
+            for (int t = 0; t < nc; ++t) seen |= cand[t] == q;
+            if (!seen) cand[nc++] = q;
+        }
+        const int sel = law_select_phase<R>(model, res, c[i], T[i], cand, nc);
+        if (sel >= 0) qid = sel;
+    }
     phi_comp[i] = qid;
 
     // constitutive velocity v = M (dG - sigma(n) kappa) (src/gpk_physics.cpp:97-146)
@@ -126,6 +141,14 @@ __device__ __forceinline__ void comp_vel_cell(
     R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
     const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
     const R mob = R(model.M0);
+    // elastic energy of the product side minus the parent's
+    // (src/gpk_physics.cpp:122-127)
+    if (model.flags & GPK_FLAG_ELASTIC) {
+        R ep = E_el[i];
+        for (int k = 0; k < 6; ++k)
+            if (phi[f[k]] == qid) { ep = E_el[f[k]]; break; }
+        dG += (ep - E_el[i]);
+    }
     // stored dislocation energy of the product side minus the parent's
     // (src/gpk_physics.cpp:131-140)
     if (model.flags & GPK_FLAG_DISLOCATION) {
@@ -144,7 +167,8 @@ __device__ __forceinline__ void finish_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
     int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
     const int32_t* __restrict__ phi, const R* __restrict__ g, const R* __restrict__ c,
-    const R* __restrict__ T, const R* __restrict__ rho, R* __restrict__ onx, R* __restrict__ ony,
+    const R* __restrict__ T, const R* __restrict__ rho, const R* __restrict__ E_el, R* __restrict__ onx,
+    R* __restrict__ ony,
     R* __restrict__ onz,
     R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
     R* __restrict__ g_comp) {
@@ -169,7 +193,7 @@ __device__ __forceinline__ void finish_cell(
     }
     onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
     comp_vel_cell<R, DIM>(d, model, i, a, z0, y0, x0, res, nxv, nyv, nzv, kap, cube_at, phi, g, c, T,
-                          rho, vraw, phi_comp, g_comp);
+                          rho, E_el, vraw, phi_comp, g_comp);
 }
 
 
@@ -181,7 +205,7 @@ __device__ __forceinline__ void geometry_cell(
     const int32_t* __restrict__ phi, const R* __restrict__ w0, const R* __restrict__ w1,
     const R* __restrict__ w2, const R* __restrict__ tab, const RowLutDesc& lutd,
     const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
-    const R* __restrict__ rho,
+    const R* __restrict__ rho, const R* __restrict__ E_el,
     R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz, R* __restrict__ okappa,
     R* __restrict__ vraw, int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     const int32_t res = phi[i];
@@ -251,7 +275,7 @@ __device__ __forceinline__ void geometry_cell(
         return phi[lin(d, zz, yy, xx)];
     };
     finish_cell<R, DIM>(d, gp, model, i, a, z0, y0, x0, res, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz, Iyz,
-                        cube_at, phi, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                        cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
 

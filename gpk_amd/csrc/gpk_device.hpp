@@ -140,6 +140,28 @@ __device__ __forceinline__ Bits128 grow5(Bits128 c, const BoxMasks& m, bool d3) 
 // ---------------------------------------------------------------------------
 // Constitutive laws (include/gpk_thermo.hpp:18-85, 229-325; src/gpk_thermo.cpp:9-30)
 // ---------------------------------------------------------------------------
+// PrecipitationThermo::G (include/gpk_thermo.hpp:358-360)
+template <class R>
+__device__ __forceinline__ R law_gibbs(const gpk_model_desc& d, int q, R cv, R Tv) {
+    return R(d.pa[q]) + R(d.pb[q]) * cv + R(d.pc2[q]) * cv * cv + R(d.pdT[q]) * (Tv - R(d.T_ref));
+}
+
+// PrecipitationThermo::select_product_phase (include/gpk_thermo.hpp:371-383)
+template <class R>
+__device__ __forceinline__ int law_select_phase(const gpk_model_desc& d, int32_t pp, R cv, R Tv,
+                                                const int32_t* cand, int nc) {
+    R best = R(1e30f);
+    int best_q = -1;
+    if (pp < 0 || pp >= d.n_phases) return -1;
+    for (int k = 0; k < nc; ++k) {
+        const int32_t q = cand[k];
+        if (q < 0 || q >= d.n_phases || q == pp) continue;
+        const R dG = law_gibbs<R>(d, q, cv, Tv) - law_gibbs<R>(d, pp, cv, Tv);
+        if (dG < best) { best = dG; best_q = q; }
+    }
+    return best_q;
+}
+
 template <class R>
 __device__ __forceinline__ R law_driving_force(const gpk_model_desc& d, int32_t pp, int32_t pq,
                                                R cv, R Tv) {
@@ -162,6 +184,9 @@ __device__ __forceinline__ R law_driving_force(const gpk_model_desc& d, int32_t 
         if (pp == 0 && pq >= 1) return -R(d.E_stored);
         if (pp >= 1 && pq == 0) return R(d.E_stored);
         return R(0);
+    case GPK_MODEL_PRECIPITATION:       // include/gpk_thermo.hpp:362-369
+        if (pp < 0 || pp >= d.n_phases || pq < 0 || pq >= d.n_phases) return R(0);
+        return law_gibbs<R>(d, pq, cv, Tv) - law_gibbs<R>(d, pp, cv, Tv);
     default:
         return R(0);
     }

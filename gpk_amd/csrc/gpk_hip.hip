@@ -272,6 +272,8 @@ struct Engine final : EngineBase {
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg;
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
+    DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
+    bool el_nonzero = false;
     // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
     // flags / list of the interface labels, hipcub scratch for the label list
     bool broad = false;
@@ -333,7 +335,8 @@ struct Engine final : EngineBase {
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
                                &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
-                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag,
+                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
+                               &eps_b, &eps_c,
                                &llist, &lcub};
         for (auto* b : all) t += b->bytes;
         return t;
@@ -354,6 +357,7 @@ struct Engine final : EngineBase {
         phi.alloc(n * 4); mask.alloc(n); phi_comp.alloc(n * 4); slot.alloc(n * 4);
         cand_id.alloc(n * 4); pid.alloc(n * 4); mark.alloc(n * 4);
         alpha.alloc(n * rs); c.alloc(n * rs); T.alloc(n * rs); g.alloc(n * rs); rho.alloc(n * rs);
+        E_el.alloc(n * rs);
         g_comp.alloc(n * rs); work1.alloc(n * rs); work2.alloc(n * rs); work3.alloc(n * rs);
         work4.alloc(n * rs);
         bandA.alloc(n * 4); bandB.alloc(n * 4);
@@ -438,6 +442,7 @@ struct Engine final : EngineBase {
         fill<R>(T, N, R(800));
         fill<R>(g, N, R(0));
         fill<R>(rho, N, R(0));
+        fill<R>(E_el, N, R(0));
         fill<R>(g_comp, N, R(0));
         HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(Scalars), stream));
         HIPCHK(hipMemsetAsync(tagctr.p, 0, 16, stream));
@@ -495,6 +500,7 @@ struct Engine final : EngineBase {
         case GPK_FIELD_T: return &T;
         case GPK_FIELD_G: return &g;
         case GPK_FIELD_RHO: return &rho;
+        case GPK_FIELD_E_EL: return &E_el;
         case GPK_FIELD_G_COMP: return &g_comp;
         case GPK_FIELD_NX: kind = 3; return &nx;
         case GPK_FIELD_NY: kind = 3; return &ny;
@@ -606,7 +612,7 @@ struct Engine final : EngineBase {
         const int grid = 8 * ((ntiles + 7) / 8);
         k_geometry_tiled<R, DIM, RG><<<grid, 256, 0, stream>>>(
             dims, gp, model, ntx, nty, ntz, phi.as<int32_t>(), mask.as<uint8_t>(), slot.as<int>(),
-            wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(),
+            wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(),
             nx.as<R>(), ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(),
             g_comp.as<R>());
     }
@@ -652,7 +658,7 @@ struct Engine final : EngineBase {
                                                             ny.as<R>(), nz.as<R>(), kappa.as<R>());
         }
         k_broad_finish<R><<<nblk(nb, 256), 256, 0, stream>>>(
-            dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(),
+            dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(), nx.as<R>(),
             ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
         HIPCHK(hipGetLastError());
     }
@@ -666,12 +672,12 @@ struct Engine final : EngineBase {
         if (dim3)
             k_geometry<R, 3, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                 dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(), ny.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(), nx.as<R>(), ny.as<R>(),
                 nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
         else
             k_geometry<R, 2, kDirect><<<nblk(nb, 256), 256, 0, stream>>>(
                 dims, gp, model, nb, bandA.as<int>(), phi.as<int32_t>(), wt0.as<R>(), wt1.as<R>(),
-                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), nx.as<R>(), ny.as<R>(),
+                wt2.as<R>(), tab.as<R>(), lutd, g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(), nx.as<R>(), ny.as<R>(),
                 nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(), g_comp.as<R>());
     }
 
@@ -708,6 +714,32 @@ struct Engine final : EngineBase {
 
     int sweep_rho() const { return (model.flags & GPK_FLAG_DISLOCATION) ? 1 : 0; }
 
+    // src/gpk_physics.cpp:10-63; depends on c only, so it runs before the
+    // fused geometry + velocity kernels that read E_el
+    void elastic() {
+        const bool on = (model.flags & GPK_FLAG_ELASTIC) && model.elastic_modulus > 0.f &&
+                        model.eigenstrain_coeff != 0.f;
+        if (!on) {
+            if (el_nonzero) {
+                k_fill_r<R><<<gsblk(N, 256), 256, 0, stream>>>(N, E_el.as<R>(), R(0));
+                el_nonzero = false;
+            }
+            return;
+        }
+        if (!eps_a.p) { eps_a.alloc(size_t(N) * sizeof(R)); eps_b.alloc(size_t(N) * sizeof(R)); eps_c.alloc(size_t(N) * sizeof(R)); }
+        const int gb = gsblk(N, 256);
+        k_elastic_eps<R><<<gb, 256, 0, stream>>>(N, c.as<R>(), model.eigenstrain_coeff, model.c_ref,
+                                                 eps_a.as<R>(), eps_b.as<R>());
+        const int iters = std::max(1, int(model.elastic_smooth_iters));
+        for (int it = 0; it < iters; ++it) {
+            k_elastic_smooth<R><<<gb, 256, 0, stream>>>(dims, dim3, eps_b.as<R>(), eps_c.as<R>());
+            eps_b.swap(eps_c);
+        }
+        k_elastic_energy<R><<<gb, 256, 0, stream>>>(N, eps_a.as<R>(), eps_b.as<R>(), model.elastic_modulus,
+                                                    E_el.as<R>());
+        el_nonzero = true;
+    }
+
     // pair projection of vraw into vn (src/gpk_physics.cpp:152-193), vmax on the side
     void project(int nb) {
         k_vmax_init<R><<<1, 1, 0, stream>>>(ds());
@@ -743,9 +775,11 @@ struct Engine final : EngineBase {
     // current labels with (eps4, eps6, M0) = theta; leaves the derived band
     // fields (mask, normals, curvature, phi_comp, vraw, vn) for these labels ----
     int velocity_chain(const double theta[3]) {
-        if (model.kind != GPK_MODEL_ISOTROPIC && model.kind != GPK_MODEL_ANISO_GROWTH)
+        if ((model.kind != GPK_MODEL_ISOTROPIC && model.kind != GPK_MODEL_ANISO_GROWTH) ||
+            (model.flags & (GPK_FLAG_ELASTIC | GPK_FLAG_DISLOCATION)))
             throw GpkError(GPK_E_UNSUPPORTED_MODEL,
-                           "velocity adjoint: isotropic or anisotropic-growth models only");
+                           "velocity adjoint: isotropic or anisotropic-growth models without "
+                           "elastic or stored-energy coupling");
         const gpk_model_desc saved = model;
         model.eps4 = float(theta[0]);
         model.eps6 = float(theta[1]);
@@ -819,6 +853,7 @@ struct Engine final : EngineBase {
     void step(float dt_req, float cap, gpk_step_stats* st) override {
         const int nb = band_compact(bandA, true);
         geo_band = nb;
+        elastic();
         // geometry + competitor + constitutive velocity
         GeomParams gp{W.rg, 2 * W.rg + 1, dx_f, W.sigma_cells};
         if (nb > 0) {
@@ -1256,6 +1291,8 @@ void gpk_model_defaults(int32_t kind, gpk_model_desc* m) {
     m->bxx = m->byy = m->bzz = 1.f;
     if (kind == GPK_MODEL_ISOTROPIC || kind == GPK_MODEL_ANISO_GROWTH) m->flags = GPK_FLAG_GRAIN_NETWORK;
     if (kind == GPK_MODEL_SOLIDIFICATION) { m->M0 = 5e-8f; m->sigma0 = 0.2f; m->eps4 = 0.04f; }
+    m->elastic_smooth_iters = 8;   // include/gpk_core.hpp:99
+    m->T_ref = 900.f;
 }
 
 int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model, gpk_handle** out) {
@@ -1267,8 +1304,10 @@ int gpk_create(const gpk_grid_desc* grid, const gpk_model_desc* model, gpk_handl
         if (1LL * grid->Nz * grid->Ny * grid->Nx >= (1LL << 31))
             throw GpkError(GPK_E_INVALID, "grid exceeds 2^31 cells");
         if (!(grid->dx > 0.f)) throw GpkError(GPK_E_INVALID, "dx must be positive");
-        if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_DISLOCATION)
+        if (model->kind < GPK_MODEL_ISOTROPIC || model->kind > GPK_MODEL_PRECIPITATION)
             throw GpkError(GPK_E_UNSUPPORTED_MODEL, "unknown model kind");
+        if (model->kind == GPK_MODEL_PRECIPITATION && (model->n_phases < 1 || model->n_phases > GPK_MAX_PHASES))
+            throw GpkError(GPK_E_INVALID, "precipitation needs 1..GPK_MAX_PHASES phases");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             throw GpkError(GPK_E_NO_DEVICE, "no HIP device visible");

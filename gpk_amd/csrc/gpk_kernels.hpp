@@ -48,13 +48,13 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
            const R* __restrict__ w0, const R* __restrict__ w1, const R* __restrict__ w2,
            const R* __restrict__ tab, RowLutDesc lutd,
            const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
-           const R* __restrict__ rho,
+           const R* __restrict__ rho, const R* __restrict__ E_el,
            R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
            R* __restrict__ okappa, R* __restrict__ vraw,
            int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nband) return;
-    geometry_cell<R, DIM, kDirect>(d, gp, model, a, band[a], phi, w0, w1, w2, tab, lutd, g, c, T, rho, onx,
+    geometry_cell<R, DIM, kDirect>(d, gp, model, a, band[a], phi, w0, w1, w2, tab, lutd, g, c, T, rho, E_el, onx,
                                    ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
@@ -107,7 +107,7 @@ __device__ __forceinline__ void tiled_cell(
     int32_t p, const R* lut, const R (*wy)[2 * RG + 1], RowMask rowmask,
     const int32_t* __restrict__ phi, const int* __restrict__ slot, const R* __restrict__ g,
     const R* __restrict__ c, const R* __restrict__ T, const R* __restrict__ rho,
-    R* __restrict__ onx, R* __restrict__ ony,
+    const R* __restrict__ E_el, R* __restrict__ onx, R* __restrict__ ony,
     R* __restrict__ onz, R* __restrict__ okappa, R* __restrict__ vraw,
     int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
@@ -152,7 +152,7 @@ __device__ __forceinline__ void tiled_cell(
         return phi[lin(d, zz, yy, xx)];
     };
     finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, p, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz,
-                        Iyz, cube_at, phi, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                        Iyz, cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
 }
 
 template <class R, int DIM, int RG>
@@ -163,7 +163,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  const R* __restrict__ w1, const R* __restrict__ w2,
                  const R* __restrict__ lut_g,
                  const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
-                 const R* __restrict__ rho,
+                 const R* __restrict__ rho, const R* __restrict__ E_el,
                  R* __restrict__ onx, R* __restrict__ ony, R* __restrict__ onz,
                  R* __restrict__ okappa, R* __restrict__ vraw,
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
@@ -278,7 +278,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                                    [&](int jz, int jy) {
                                        return rows[(jz * S::SY + jy) * GEO_MAXRES] >> lx;
                                    },
-                                   phi, slot, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                                   phi, slot, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
         }
     }
     // residents that did not fit the hash: window rows read from the grid
@@ -299,7 +299,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                                            m |= 1ull << jx;
                                    return m;
                                },
-                               phi, slot, g, c, T, rho, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                               phi, slot, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
     }
 }
 
@@ -1655,6 +1655,41 @@ __global__ void k_fill_u8(int n, uint8_t* p, uint8_t v) {
 template <class R>
 __global__ void k_fill_r(int n, R* p, R v) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+// elastic coupling (src/gpk_physics.cpp:10-63): eigenstrain, Jacobi-type
+// smoothing sweeps, energy density 0.5 E (eps* - smooth(eps*))^2
+template <class R>
+__global__ void k_elastic_eps(int n, const R* __restrict__ c, float coeff, float c_ref,
+                              R* __restrict__ eps_star, R* __restrict__ eps_smooth) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const R e = R(coeff) * (c[i] - R(c_ref));
+        eps_star[i] = e;
+        eps_smooth[i] = e;
+    }
+}
+template <class R>
+__global__ void k_elastic_smooth(Dims d, int dim3, const R* __restrict__ src, R* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        int f[6];
+        faces(d, i, f);
+        R avg = src[f[0]] + src[f[1]] + src[f[2]] + src[f[3]];
+        if (dim3) {
+            avg += src[f[4]] + src[f[5]];
+            avg /= R(6);
+        } else {
+            avg /= R(4);
+        }
+        dst[i] = R(0.5) * src[i] + R(0.5) * avg;
+    }
+}
+template <class R>
+__global__ void k_elastic_energy(int n, const R* __restrict__ eps_star, const R* __restrict__ eps_smooth,
+                                 float modulus, R* __restrict__ E_el) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const R dd = eps_star[i] - eps_smooth[i];
+        E_el[i] = R(0.5) * R(modulus) * dd * dd;
+    }
 }
 
 template <class R>

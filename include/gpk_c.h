@@ -69,7 +69,8 @@ enum gpk_field {
     GPK_FIELD_PHI_COMP = 11,  /* I32  competitor label               */
     GPK_FIELD_G_COMP   = 12,  /* R    competitor orientation         */
     GPK_FIELD_MASK     = 13,  /* U8   interface mask                 */
-    GPK_FIELD_COUNT    = 14
+    GPK_FIELD_E_EL     = 14,  /* R    elastic energy density (GPK_FLAG_ELASTIC) */
+    GPK_FIELD_COUNT    = 15
 };
 
 /* ---- constitutive models (POD replacement of the ThermoModel plugin) --- */
@@ -80,14 +81,18 @@ enum gpk_model_kind {
     GPK_MODEL_TENSOR_ANISO   = 3,  /* TensorAnisotropyThermo  include/gpk_thermo.hpp:229-263 */
     GPK_MODEL_ANISO_OSTWALD  = 4,  /* AnisoOstwaldThermo      include/gpk_thermo.hpp:284-325 */
     GPK_MODEL_RECRYSTALLISATION = 5, /* RecrystallisationThermo include/gpk_thermo.hpp:90-111 */
-    GPK_MODEL_DISLOCATION    = 6   /* DislocationThermo       include/gpk_thermo.hpp:135-170 */
+    GPK_MODEL_DISLOCATION    = 6,  /* DislocationThermo       include/gpk_thermo.hpp:135-170 */
+    GPK_MODEL_PRECIPITATION  = 7   /* PrecipitationThermo     include/gpk_thermo.hpp:333-387 */
 };
+
+#define GPK_MAX_PHASES 8           /* PrecipitationThermo phases held by the descriptor */
 
 /* flag bits of gpk_model_desc.flags (ThermoModel fields, include/gpk_core.hpp:72-123) */
 #define GPK_FLAG_GRAIN_NETWORK    0x1u
 #define GPK_FLAG_SOLUTE_REJECTION 0x2u
 #define GPK_FLAG_STOCHASTIC       0x4u
 #define GPK_FLAG_DISLOCATION      0x8u   /* dislocation_enabled: rho coupling, sweep, recovery */
+#define GPK_FLAG_ELASTIC          0x10u  /* elastic_enabled: smoothed-eigenstrain energy coupling */
 
 typedef struct gpk_model_desc {
     int32_t  kind;          /* gpk_model_kind */
@@ -110,6 +115,15 @@ typedef struct gpk_model_desc {
     float disloc_coeff;     /* E_rho = disloc_coeff * rho */
     float rho_rex;          /* density left behind a swept boundary */
     float recovery_rate;    /* d(rho)/dt = -recovery_rate * rho */
+    /* elastic coupling (src/gpk_physics.cpp:10-63), GPK_FLAG_ELASTIC:
+     * eps* = eigenstrain_coeff (c - c_ref), smoothed elastic_smooth_iters times */
+    float elastic_modulus, eigenstrain_coeff, c_ref;
+    int32_t elastic_smooth_iters;
+    /* PrecipitationThermo: G_q(c, T) = a_q + b_q c + c2_q c^2 + dT_q (T - T_ref),
+     * q < n_phases <= GPK_MAX_PHASES; the product phase is selected per cell */
+    int32_t n_phases;
+    float T_ref;
+    float pa[GPK_MAX_PHASES], pb[GPK_MAX_PHASES], pc2[GPK_MAX_PHASES], pdT[GPK_MAX_PHASES];
 } gpk_model_desc;
 
 /* Fills *m with the defaults of the reference ThermoModel base and the given

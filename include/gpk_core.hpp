@@ -108,15 +108,21 @@ struct ThermoModel {
         d->flags = (grain_network ? GPK_FLAG_GRAIN_NETWORK : 0u) |
                    (solute_rejection ? GPK_FLAG_SOLUTE_REJECTION : 0u) |
                    (stochastic ? GPK_FLAG_STOCHASTIC : 0u) |
-                   (dislocation_enabled ? GPK_FLAG_DISLOCATION : 0u);
+                   (dislocation_enabled ? GPK_FLAG_DISLOCATION : 0u) |
+                   (elastic_enabled ? GPK_FLAG_ELASTIC : 0u);
+        d->elastic_modulus = elastic_modulus;
+        d->eigenstrain_coeff = eigenstrain_coeff;
+        d->c_ref = c_ref;
+        d->elastic_smooth_iters = elastic_smooth_iters;
         d->disloc_coeff = disloc_coeff;
         d->rho_rex = rho_rex;
         d->recovery_rate = recovery_rate;
     }
     // couplings that exist in the reference but not on the device path
-    // (the stored-energy coupling is on it: GPK_FLAG_DISLOCATION)
+    // (stored-energy and elastic couplings are on it; phase selection only
+    // through PrecipitationThermo; magnetostrictive eigenstrain is not)
     bool couplings_off() const {
-        return !phase_selection && !elastic_enabled && !magnetic_enabled && Q == 0.f;
+        return !phase_selection && !magnetic_enabled && lambda_s == 0.f && Q == 0.f;
     }
 };
 

@@ -9,6 +9,9 @@
  */
 #include "gpk_core.hpp"
 
+#include <stdexcept>
+#include <vector>
+
 namespace gpk {
 
 // Isotropic grain growth, dG = 0, M = M0 (include/gpk_thermo.hpp:18-27)
@@ -145,6 +148,41 @@ struct DislocationThermo : ThermoModel {
     bool device_descriptor(gpk_model_desc* d) const override {
         if (!couplings_off()) return false;
         fill_common(d, GPK_MODEL_DISLOCATION);
+        return true;
+    }
+};
+
+// include/gpk_thermo.hpp:333-387: phases 0..n-1 with polynomial Gibbs
+// energies G_q(c, T) = a_q + b_q c + c2_q c^2 + dT_q (T - T_ref); the product
+// phase of a cell is chosen by local Gibbs minimisation on the device.
+struct PrecipitationThermo : ThermoModel {
+    int n_phases = 0;
+    float T_ref = 900.f;
+    std::vector<float> a, b, c2, dT;
+    PrecipitationThermo(float M0_val, float sigma0_val, int n_phases_val, const std::vector<float>& a_val,
+                        const std::vector<float>& b_val, const std::vector<float>& c2_val,
+                        const std::vector<float>& dT_val, float T_ref_val = 900.f)
+        : n_phases(n_phases_val), T_ref(T_ref_val), a(a_val), b(b_val), c2(c2_val), dT(dT_val) {
+        M0 = M0_val;
+        sigma0 = sigma0_val;
+        eps4 = 0.f;
+        phase_selection = true;
+        if (int(a.size()) != n_phases || int(b.size()) != n_phases || int(c2.size()) != n_phases ||
+            int(dT.size()) != n_phases)
+            throw std::runtime_error("PrecipitationThermo: coefficient size mismatch");
+    }
+    float G(int q, float c_val, float T_val) const {
+        return a[q] + b[q] * c_val + c2[q] * c_val * c_val + dT[q] * (T_val - T_ref);
+    }
+    bool device_descriptor(gpk_model_desc* d) const override {
+        if (magnetic_enabled || lambda_s != 0.f || Q != 0.f || n_phases < 1 || n_phases > GPK_MAX_PHASES)
+            return false;
+        fill_common(d, GPK_MODEL_PRECIPITATION);
+        d->n_phases = n_phases;
+        d->T_ref = T_ref;
+        for (int q = 0; q < n_phases; ++q) {
+            d->pa[q] = a[q]; d->pb[q] = b[q]; d->pc2[q] = c2[q]; d->pdT[q] = dT[q];
+        }
         return true;
     }
 };

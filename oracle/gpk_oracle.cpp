@@ -226,9 +226,30 @@ struct Law {
             if (pp == 0 && pq >= 1) return -R(d.E_stored);
             if (pp >= 1 && pq == 0) return R(d.E_stored);
             return R(0);
+        case GPK_MODEL_PRECIPITATION:       // include/gpk_thermo.hpp:362-369
+            if (pp < 0 || pp >= d.n_phases || pq < 0 || pq >= d.n_phases) return R(0);
+            return gibbs(pq, cv, Tv) - gibbs(pp, cv, Tv);
         default:
             return R(0);
         }
+    }
+
+    // include/gpk_thermo.hpp:358-360
+    R gibbs(int q, R cv, R Tv) const {
+        return R(d.pa[q]) + R(d.pb[q]) * cv + R(d.pc2[q]) * cv * cv + R(d.pdT[q]) * (Tv - R(d.T_ref));
+    }
+    // include/gpk_thermo.hpp:371-383
+    int select_phase(int32_t pp, R cv, R Tv, const int32_t* cand, int nc) const {
+        R best = R(1e30f);
+        int best_q = -1;
+        if (pp < 0 || pp >= d.n_phases) return -1;
+        for (int k = 0; k < nc; ++k) {
+            const int32_t q = cand[k];
+            if (q < 0 || q >= d.n_phases || q == pp) continue;
+            const R dG = gibbs(q, cv, Tv) - gibbs(pp, cv, Tv);
+            if (dG < best) { best = dG; best_q = q; }
+        }
+        return best_q;
     }
 
     R interface_composition(int32_t pp, int32_t pq, R cp, R cq) const {
@@ -312,7 +333,7 @@ struct Grid {
     int proj_r = 1, remap_r = 1;
 
     std::vector<int32_t> phi, phi_comp;
-    std::vector<R> alpha, c, T, g, rho, nx, ny, nz, kappa, vn, g_comp;
+    std::vector<R> alpha, c, T, g, rho, nx, ny, nz, kappa, vn, g_comp, E_el;
     std::vector<uint8_t> mask;
     std::vector<int> active;
     std::vector<R> alpha_new, eps_star, eps_smooth, solid_old;
@@ -340,7 +361,7 @@ struct Grid {
         alpha.assign(N, R(1)); c.assign(N, R(0)); T.assign(N, R(800)); g.assign(N, R(0));
         rho.assign(N, R(0));
         nx.assign(N, R(0)); ny.assign(N, R(0)); nz.assign(N, R(0));
-        kappa.assign(N, R(0)); vn.assign(N, R(0)); g_comp.assign(N, R(0));
+        kappa.assign(N, R(0)); vn.assign(N, R(0)); g_comp.assign(N, R(0)); E_el.assign(N, R(0));
         mask.assign(N, 0);
         alpha_new.assign(N, R(0));
         eps_star.assign(N, R(0)); eps_smooth.assign(N, R(0));
@@ -644,6 +665,19 @@ struct Grid {
         for (int a = 0; a < na; ++a) {
             const int i = active[a];
             int f[6]; faces(i, f);
+            // src/gpk_physics.cpp:79-95
+            if (law.d.kind == GPK_MODEL_PRECIPITATION) {
+                int32_t cand[6]; int nc = 0;
+                for (int k = 0; k < 6; ++k) {
+                    const int32_t q = phi[f[k]];
+                    if (q == phi[i]) continue;
+                    bool seen = false;
+                    for (int t = 0; t < nc; ++t) if (cand[t] == q) { seen = true; break; }
+                    if (!seen) cand[nc++] = q;
+                }
+                const int sel = law.select_phase(phi[i], c[i], T[i], cand, nc);
+                if (sel >= 0) phi_comp[i] = sel;
+            }
             const int32_t q = phi_comp[i];
             const R gp = g[i];
             R gq = gp, cq = c[i];
@@ -654,6 +688,13 @@ struct Grid {
             R dG = law.driving_force(phi[i], q, ci, T[i]);
             const R sig = law.energy(nx[i], ny[i], nz[i], gp);
             const R mob = R(law.d.M0);
+            // src/gpk_physics.cpp:122-127
+            if (law.d.flags & GPK_FLAG_ELASTIC) {
+                R ep = E_el[i];
+                for (int k = 0; k < 6; ++k)
+                    if (phi[f[k]] == q) { ep = E_el[f[k]]; break; }
+                dG += (ep - E_el[i]);
+            }
             // src/gpk_physics.cpp:131-140
             if (law.d.flags & GPK_FLAG_DISLOCATION) {
                 R rp = rho[i];
@@ -1162,6 +1203,7 @@ struct Grid {
         interface_mask();
         st.band = static_cast<int64_t>(active.size());
         geometry();
+        elastic();
         velocities();
         projection();
         const R stable = max_dt();
@@ -1176,6 +1218,32 @@ struct Grid {
         time += double(dt);
         ++step_id;
         return dt;
+    }
+
+    // -- src/gpk_physics.cpp:10-63 -------------------------------------------------
+    void elastic() {
+        const auto& d = law.d;
+        if (!(d.flags & GPK_FLAG_ELASTIC) || d.elastic_modulus <= 0.f || d.eigenstrain_coeff == 0.f) {
+            std::fill(E_el.begin(), E_el.end(), R(0));
+            return;
+        }
+        std::vector<R> es(N), sm(N), tmp(N);
+        for (int i = 0; i < N; ++i) { es[i] = R(d.eigenstrain_coeff) * (c[i] - R(d.c_ref)); sm[i] = es[i]; }
+        const int iters = std::max(1, int(d.elastic_smooth_iters));
+        for (int it = 0; it < iters; ++it) {
+            for (int i = 0; i < N; ++i) {
+                int f[6]; faces(i, f);
+                R avg = sm[f[0]] + sm[f[1]] + sm[f[2]] + sm[f[3]];
+                if (Nz > 1) { avg += sm[f[4]] + sm[f[5]]; avg /= R(6); }
+                else avg /= R(4);
+                tmp[i] = R(0.5) * sm[i] + R(0.5) * avg;
+            }
+            sm.swap(tmp);
+        }
+        for (int i = 0; i < N; ++i) {
+            const R dd = es[i] - sm[i];
+            E_el[i] = R(0.5) * R(d.elastic_modulus) * dd * dd;
+        }
     }
 
     // -- src/gpk_transport.cpp:142-152 ---------------------------------------------
@@ -1351,7 +1419,7 @@ struct Impl final : Base {
         case GPK_FIELD_RHO: return &G.rho; case GPK_FIELD_NX: return &G.nx;
         case GPK_FIELD_NY: return &G.ny; case GPK_FIELD_NZ: return &G.nz;
         case GPK_FIELD_KAPPA: return &G.kappa; case GPK_FIELD_VN: return &G.vn;
-        case GPK_FIELD_G_COMP: return &G.g_comp;
+        case GPK_FIELD_G_COMP: return &G.g_comp; case GPK_FIELD_E_EL: return &G.E_el;
         default: return nullptr;
         }
     }

@@ -43,6 +43,12 @@ std::shared_ptr<gpk::ThermoModel> make_model(const gpk_model_desc& m) {
                                                        m.c_sat, m.Omega); break;
     case GPK_MODEL_RECRYSTALLISATION:
         t = std::make_shared<gpk::RecrystallisationThermo>(m.M0, m.sigma0, m.E_stored); break;
+    case GPK_MODEL_PRECIPITATION: {
+        const int n = m.n_phases;
+        std::vector<float> a(m.pa, m.pa + n), b(m.pb, m.pb + n), c2(m.pc2, m.pc2 + n), dT(m.pdT, m.pdT + n);
+        t = std::make_shared<gpk::PrecipitationThermo>(m.M0, m.sigma0, n, a, b, c2, dT, m.T_ref);
+        break;
+    }
     case GPK_MODEL_DISLOCATION:
         // constructed with unit mu, b; the coefficient is taken from the descriptor
         t = std::make_shared<gpk::DislocationThermo>(m.M0, m.sigma0, 1.f, 1.f); break;
@@ -60,6 +66,11 @@ std::shared_ptr<gpk::ThermoModel> make_model(const gpk_model_desc& m) {
     t->disloc_coeff = m.disloc_coeff;
     t->rho_rex = m.rho_rex;
     t->recovery_rate = m.recovery_rate;
+    t->elastic_enabled = (m.flags & GPK_FLAG_ELASTIC) != 0;
+    t->elastic_modulus = m.elastic_modulus;
+    t->eigenstrain_coeff = m.eigenstrain_coeff;
+    t->c_ref = m.c_ref;
+    t->elastic_smooth_iters = m.elastic_smooth_iters;
     return t;
 }
 
@@ -113,6 +124,7 @@ int rf_get(void* p, int f, void* dst, size_t b) {
     case GPK_FIELD_T: return copy_out(g.T, dst, b);
     case GPK_FIELD_G: return copy_out(g.g, dst, b);
     case GPK_FIELD_RHO: return copy_out(g.rho, dst, b);
+    case GPK_FIELD_E_EL: return copy_out(g.E_el, dst, b);
     case GPK_FIELD_NX: return copy_out(g.nx_f, dst, b);
     case GPK_FIELD_NY: return copy_out(g.ny_f, dst, b);
     case GPK_FIELD_NZ: return copy_out(g.nz_f, dst, b);

@@ -12,7 +12,8 @@ import pytest
 import gpk_amd
 from gpk_amd import _abi
 from oracle import oracle as O
-from tests.util import FIELDS_ALL, dislocation_rho, rex_setup, solidification_setup
+from tests.util import (FIELDS_ALL, dislocation_rho, elastic_solidification_setup, precipitation_setup,
+                        rex_setup, solidification_setup)
 
 pytestmark = pytest.mark.gpu
 
@@ -100,6 +101,20 @@ def test_dislocation_sweep_and_recovery_2d(mode):
     model = _abi.dislocation(1e-9, 0.5, 4e10, 2.5e-10, 0.5, recovery_rate=50.0, rho_rex=1e12)
     o, G = pair((1, 80, 80), model, 0.0, 1.5, mode, {"phi": rex_setup(), "rho": dislocation_rho()}, dx=1e-6)
     lockstep(o, G, 60, fields=FIELDS_ALL + ("rho",), check_every=5)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp64"])
+def test_precipitation_phase_selection_2d(mode):
+    model, init = precipitation_setup()
+    o, G = pair((1, 80, 80), model, 2e-9, 1.5, mode, init, dx=0.5e-6)
+    lockstep(o, G, 50, check_every=5)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_elastic_coupling_2d(mode):
+    model, init = elastic_solidification_setup()
+    o, G = pair((1, 80, 80), model, 1e-9, 1.5, mode, init, dx=1e-6)
+    lockstep(o, G, 50, fields=FIELDS_ALL + ("E_el",), check_every=5)
 
 
 def test_aniso_ostwald_lamella_2d_strict32():

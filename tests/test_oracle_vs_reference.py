@@ -7,7 +7,8 @@ import pytest
 
 from gpk_amd import _abi
 from oracle import oracle as O
-from tests.util import dislocation_rho, rex_setup, solidification_setup
+from tests.util import (dislocation_rho, elastic_solidification_setup, precipitation_setup, rex_setup,
+                        solidification_setup)
 
 pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built")
 
@@ -127,3 +128,25 @@ def test_dislocation_sweep_and_recovery_2d():
     assert accepted > 0
     rho = o.get("rho")
     assert rho.min() < 1e12 and rho.max() < 1.5e14   # swept and recovered
+
+
+def test_precipitation_phase_selection_2d():
+    # PrecipitationThermo: per-cell lowest-G product phase replaces the
+    # geometric competitor (src/gpk_physics.cpp:79-95)
+    model, init = precipitation_setup()
+    run_pair((1, 80, 80), model, 2e-9, 1.5, 50, init, dx=0.5e-6)
+
+
+def test_elastic_coupling_2d():
+    # eigenstrain smoothing + energy coupling (src/gpk_physics.cpp:10-63, 122-127)
+    model, init = elastic_solidification_setup()
+    o = O.OracleGrid(1, 80, 80, 1e-6, model, 1e-9, 1.5, "strict32")
+    r = O.RefGrid(1, 80, 80, 1e-6, model, 1e-9, 1.5)
+    for name, arr in init.items():
+        o.set(name, arr)
+        r.set(name, arr)
+    for s in range(50):
+        assert o.step() == r.step(), s
+        for f in FIELDS + ("E_el",):
+            assert np.array_equal(o.get(f), r.get(f)), (f, s)
+    assert o.get("E_el").max() > 0

@@ -84,3 +84,28 @@ def dislocation_rho(N=80):
     """Dislocation density varying along x, 1e14 m^-2 +- 50 %."""
     x = np.arange(N)[None, :].repeat(N, 0)
     return (1e14 * (1 + 0.5 * np.sin(x / 7.0))).astype(np.float32)
+
+
+def precipitation_setup(N=80):
+    """Matrix phase 0 with discs of phases 1 and 2; c varies along x, T = 880 K
+    (inputs/precipitation.ini coefficients)."""
+    y, x = np.mgrid[0:N, 0:N]
+    phi = rex_setup(N)
+    phi[phi == 3] = 1
+    c = (0.06 + 0.01 * np.sin(x / 9.0)).astype(np.float32)
+    T = np.full((N, N), 880.0, np.float32)
+    model = _abi.precipitation(2e-8, 0.18, [0.0, -8.0e6, -7.5e6], [0.0, 2.0e7, -1.0e7],
+                               [0.0, 2.5e8, 1.8e8], [0.0, 1.0e4, 5.0e3], 900.0)
+    return model, {"phi": phi, "c": c, "T": T}
+
+
+def elastic_solidification_setup(N=80):
+    """Solid disc r = 8 with c = 0.006 in liquid c = 0.02, 20 K undercooled;
+    elastic coupling E = 5e9, eigenstrain 0.05 (c - 0.02), 12 smoothing sweeps."""
+    y, x = np.mgrid[0:N, 0:N]
+    phi = (((y - 40) ** 2 + (x - 40) ** 2) <= 64).astype(np.int32)
+    c = np.full((N, N), 0.02, np.float32)
+    c[phi == 1] = 0.006
+    T = np.full((N, N), 903.0, np.float32)
+    model = _abi.with_elastic(_abi.solidification(M0=5e-8, sigma0=0.2, eps4=0.0), 5e9, 0.05, 0.02, 12)
+    return model, {"phi": phi, "c": c, "T": T}
