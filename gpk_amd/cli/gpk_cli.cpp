@@ -2,8 +2,9 @@
 //
 // Runs the in-scope models of the reference CLI (demos/gpk.cpp):
 // [run] model = grain_growth (run_grain_growth, 195-282),
-// [run] model = solidification (run_solidification, 288-391) and
-// [run] model = recrystallisation (run_recrystallisation, 477-555), with the same
+// [run] model = solidification (run_solidification, 288-391),
+// [run] model = recrystallisation (run_recrystallisation, 477-555) and
+// [run] model = precipitation (run_precipitation, 703-797), with the same
 // deck keys, initial conditions, progress tables and snapshot files
 // ([output] format = csv | vtk | ppm | all, dir, prefix, write_initial).
 // Optional keys: [gpu] precision = strict32 | fp32 | fp64, [gpu] device.
@@ -320,14 +321,22 @@ int run_solidification(const Deck& cfg) {
     const float L = cfg.f("thermo", "L");
     const float c0 = cfg.f("initial", "c0"), dT = cfg.f("initial", "undercooling");
     const int seed_r = cfg.i("initial", "seed_radius", 4);
-    if (cfg.b("thermo", "elastic_enabled", false))
-        throw std::runtime_error("elastic coupling has no device form in this build");
+    const bool elastic_enabled = cfg.b("thermo", "elastic_enabled", false);
+    const float elastic_modulus = cfg.f("thermo", "elastic_modulus", 0.0f);
+    const float eigenstrain_coeff = cfg.f("thermo", "eigenstrain_coeff", 0.0f);
+    const int elastic_iters = cfg.i("thermo", "elastic_smooth_iters", 8);
+    const float c_ref = cfg.f("thermo", "c_ref", c0);
 
     std::printf("=== GPK Solidification (MI355X) ===\n");
     std::printf("Grid %d×%d×%d, dx=%.2f µm\n", Nz, Ny, Nx, dx * 1e6f);
     auto thermo = std::make_shared<SolidificationThermo>(M0, sig0, eps4, m_liq, k_part, T_melt, L);
     thermo->stochastic = cfg.b("thermo", "stochastic", false);
     thermo->noise_scale = cfg.f("thermo", "noise_scale", 1.0f);
+    thermo->elastic_enabled = elastic_enabled;
+    thermo->elastic_modulus = elastic_modulus;
+    thermo->eigenstrain_coeff = eigenstrain_coeff;
+    thermo->c_ref = c_ref;
+    thermo->elastic_smooth_iters = elastic_iters;
     thermo->solute_rejection = cfg.b("thermo", "solute_rejection", false);
     thermo->k_partition = k_part;
     if (thermo->solute_rejection) std::printf("Solute partitioning enabled, k = %.3f\n", double(k_part));
@@ -339,6 +348,11 @@ int run_solidification(const Deck& cfg) {
     const float R_star = sig0 * T_melt / (L * dT);
     std::printf("Gibbs-Thomson R* = %.3f µm\n", R_star * 1e6f);
     std::printf("Seed radius      = %.1f µm (must be > R*)\n\n", seed_r * dx * 1e6f);
+    if (thermo->stochastic)
+        std::printf("Stochastic kinetics enabled, noise_scale = %.3f\n", double(thermo->noise_scale));
+    if (elastic_enabled)
+        std::printf("Elastic coupling enabled, modulus = %.3e J/m^3, coeff = %.3e\n\n", double(elastic_modulus),
+                    double(eigenstrain_coeff));
     const int cz = Nz / 2, cy = Ny / 2, cx = Nx / 2;
     for (int dz = -seed_r; dz <= seed_r; ++dz)
         for (int dy = -seed_r; dy <= seed_r; ++dy)
@@ -514,6 +528,98 @@ int run_recrystallisation(const Deck& cfg) {
     return 0;
 }
 
+// demos/gpk.cpp:693-701
+std::vector<float> parse_csv_floats(const std::string& s) {
+    std::vector<float> out;
+    std::stringstream ss(s);
+    std::string tok;
+    while (std::getline(ss, tok, ','))
+        if (!tok.empty()) out.push_back(std::stof(tok));
+    return out;
+}
+
+// demos/gpk.cpp:703-797
+int run_precipitation(const Deck& cfg) {
+    const int Nz = cfg.i("grid", "Nz"), Ny = cfg.i("grid", "Ny"), Nx = cfg.i("grid", "Nx");
+    const float dx = cfg.f("grid", "dx");
+    const double t_end = cfg.d("time", "t_end");
+    const int nsnap = cfg.i("time", "n_snaps");
+    const float M0 = cfg.f("thermo", "M0"), sig0 = cfg.f("thermo", "sigma0");
+    const float D = cfg.f("thermo", "D", 0.f);
+    const int n_phases = cfg.i("thermo", "n_phases");
+    const float T_ref = cfg.f("thermo", "T_ref", 900.f);
+    const std::vector<float> a = parse_csv_floats(cfg.str("thermo", "a"));
+    const std::vector<float> b = parse_csv_floats(cfg.str("thermo", "b"));
+    const std::vector<float> c2 = parse_csv_floats(cfg.str("thermo", "c2"));
+    const std::vector<float> dT = parse_csv_floats(cfg.str("thermo", "dT"));
+    const float c0 = cfg.f("initial", "c0"), T0 = cfg.f("initial", "T0");
+    const int n_seeds = cfg.i("initial", "n_seeds");
+    const int seed_r = cfg.i("initial", "seed_radius", 2);
+    const int seed = cfg.i("initial", "seed", 123);
+
+    std::printf("=== GPK Multi-phase Precipitation (MI355X) ===\n");
+    std::printf("Grid %d×%d×%d, phases=%d, dx=%.2f µm\n\n", Nz, Ny, Nx, n_phases, dx * 1e6f);
+    auto thermo = std::make_shared<PrecipitationThermo>(M0, sig0, n_phases, a, b, c2, dT, T_ref);
+    GPKGrid grid(Nz, Ny, Nx, dx, thermo, D, cfg.f("geometry", "sigma_cells", 2.f), precision_of(cfg),
+                 cfg.i("gpu", "device", 0));
+    std::fill(grid.phi.begin(), grid.phi.end(), 0);
+    std::fill(grid.alpha.begin(), grid.alpha.end(), 1.f);
+    std::fill(grid.c.begin(), grid.c.end(), c0);
+    std::fill(grid.T.begin(), grid.T.end(), T0);
+    std::mt19937 rng(static_cast<std::mt19937::result_type>(seed));
+    std::uniform_int_distribution<int> rz(seed_r, std::max(seed_r, Nz - seed_r - 1));
+    std::uniform_int_distribution<int> ry(seed_r, std::max(seed_r, Ny - seed_r - 1));
+    std::uniform_int_distribution<int> rx(seed_r, std::max(seed_r, Nx - seed_r - 1));
+    std::uniform_int_distribution<int> rp(1, std::max(1, n_phases - 1));
+    for (int n = 0; n < n_seeds; ++n) {
+        const int cz = Nz > 1 ? rz(rng) : 0;
+        const int cy = ry(rng);
+        const int cx = rx(rng);
+        const int p = rp(rng);
+        bool overlap = false;
+        const int z_guard = Nz > 1 ? seed_r + 1 : 0;
+        for (int dz = -z_guard; dz <= z_guard && !overlap; ++dz)
+            for (int dy = -seed_r - 1; dy <= seed_r + 1 && !overlap; ++dy)
+                for (int ddx = -seed_r - 1; ddx <= seed_r + 1 && !overlap; ++ddx) {
+                    const int zz = cz + dz, yy = cy + dy, xx = cx + ddx;
+                    if (zz < 0 || zz >= Nz || yy < 0 || yy >= Ny || xx < 0 || xx >= Nx) continue;
+                    if (grid.phi[grid.idx(zz, yy, xx)] != 0) overlap = true;
+                }
+        if (overlap) continue;
+        const int z_radius = Nz > 1 ? seed_r : 0;
+        for (int dz = -z_radius; dz <= z_radius; ++dz)
+            for (int dy = -seed_r; dy <= seed_r; ++dy)
+                for (int ddx = -seed_r; ddx <= seed_r; ++ddx) {
+                    if (dz * dz + dy * dy + ddx * ddx > seed_r * seed_r) continue;
+                    const int zz = cz + dz, yy = cy + dy, xx = cx + ddx;
+                    if (zz < 0 || zz >= Nz || yy < 0 || yy >= Ny || xx < 0 || xx >= Nx) continue;
+                    grid.phi[grid.idx(zz, yy, xx)] = p;
+                }
+    }
+    grid.mark_dirty();
+    Writer writer(cfg);
+    const bool write_initial = cfg.b("output", "write_initial", false);
+    const int off = write_initial ? 1 : 0;
+    if (write_initial) writer.write(grid, 0, grid.time);
+    const double snap_dt = t_end / nsnap;
+    std::printf("%-10s %-12s %-12s %-12s\n", "Time[s]", "Phase0", "Phase1", "Phase2+");
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    for (int snap = 0; snap < nsnap; ++snap) {
+        grid.run(snap_dt * (snap + 1), -1.f);
+        grid.sync_host();
+        std::unordered_map<int32_t, int> cnt;
+        for (auto p : grid.phi) cnt[p]++;
+        const float f0 = float(cnt[0]) / float(grid.N());
+        const float f1 = float(cnt[1]) / float(grid.N());
+        const float f2p = std::max(0.f, 1.f - f0 - f1);
+        std::printf("%-10.4f %-12.4f %-12.4f %-12.4f\n", grid.time, f0, f1, f2p);
+        writer.write(grid, snap + off, grid.time);
+    }
+    const double wall = std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+    std::printf("\nWall time: %.2f s\n", wall);
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
@@ -527,8 +633,9 @@ int main(int argc, char** argv) {
         if (model == "grain_growth") return run_grain_growth(cfg);
         if (model == "solidification") return run_solidification(cfg);
         if (model == "recrystallisation") return run_recrystallisation(cfg);
+        if (model == "precipitation") return run_precipitation(cfg);
         std::fprintf(stderr, "Unknown or unsupported model '%s' (this build runs grain_growth, "
-                             "solidification and recrystallisation)\n", model.c_str());
+                             "solidification, recrystallisation and precipitation)\n", model.c_str());
         return 1;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "Error: %s\n", e.what());

@@ -21,7 +21,8 @@ Fixtures (inputs and expected outputs only):
   cli_*.npz          a deck (text, output dir templated as {out}) and every
                      snapshot file the reference CLI wrote for it (CSV, VTK and
                      PPM bytes), demos/gpk.cpp run_grain_growth (195-282),
-                     run_solidification (288-391), run_recrystallisation (477-555)
+                     run_solidification (288-391), run_recrystallisation (477-555),
+                     run_precipitation (703-797)
 """
 import argparse
 import os
@@ -214,6 +215,73 @@ Jdot = 5.0e14
 seed = 42
 [geometry]
 sigma_cells = 1.5
+[output]
+format = all
+dir = {out}
+write_initial = true
+""",
+    "cli_precipitation_2d": """[run]
+model = precipitation
+[grid]
+Nz = 1
+Ny = 64
+Nx = 64
+dx = 0.5e-6
+[time]
+t_end = 3.0e-5
+n_snaps = 2
+[thermo]
+M0 = 2.0e-8
+sigma0 = 0.18
+D = 2.0e-9
+n_phases = 3
+T_ref = 900.0
+a = 0.0, -8.0e6, -7.5e6
+b = 0.0,  2.0e7, -1.0e7
+c2 = 0.0,  2.5e8,  1.8e8
+dT = 0.0,  1.0e4,  5.0e3
+[initial]
+c0 = 0.06
+T0 = 880.0
+n_seeds = 12
+seed_radius = 2
+seed = 123
+[geometry]
+sigma_cells = 1.5
+[output]
+format = all
+dir = {out}
+write_initial = true
+""",
+    "cli_solidification_elastic_2d": """[run]
+model = solidification
+[grid]
+Nz = 1
+Ny = 64
+Nx = 64
+dx = 0.5e-6
+[time]
+t_end = 2.0e-4
+n_snaps = 2
+[thermo]
+M0 = 5.0e-8
+sigma0 = 0.20
+eps4 = 0.0
+D = 3.0e-9
+m_liq = -500.0
+k_part = 0.3
+T_melt = 933.0
+L = 1.0e8
+elastic_enabled = true
+elastic_modulus = 5.0e9
+eigenstrain_coeff = 0.05
+c_ref = 0.02
+elastic_smooth_iters = 12
+solute_rejection = true
+[initial]
+c0 = 0.02
+undercooling = 10.0
+seed_radius = 4
 [output]
 format = all
 dir = {out}

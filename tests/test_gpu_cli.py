@@ -19,7 +19,8 @@ from tests.util import GOLDEN, REPO
 pytestmark = pytest.mark.gpu
 
 CLI = os.path.join(REPO, "gpk_amd", "gpk")
-EXACT = ("cli_grain_growth_2d", "cli_grain_growth_3d", "cli_solidification_3d", "cli_recrystallisation_2d")
+EXACT = ("cli_grain_growth_2d", "cli_grain_growth_3d", "cli_solidification_3d", "cli_recrystallisation_2d",
+         "cli_precipitation_2d", "cli_solidification_elastic_2d")
 
 
 def first_difference(a, b):
