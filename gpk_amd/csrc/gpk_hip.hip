@@ -273,6 +273,10 @@ struct Engine final : EngineBase {
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
+    // remap-score cache (tiled scoring path): per-cell gain / best label,
+    // validity and the dirty marks left by label writes
+    DevBuf rs_gain, rs_best, rs_valid, rs_dirty;
+    bool rs_reset = true;
     bool el_nonzero = false;
     // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
     // flags / list of the interface labels, hipcub scratch for the label list
@@ -336,6 +340,7 @@ struct Engine final : EngineBase {
                                &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
                                &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
+                               &rs_gain, &rs_best, &rs_valid, &rs_dirty,
                                &eps_b, &eps_c,
                                &llist, &lcub};
         for (auto* b : all) t += b->bytes;
@@ -358,6 +363,7 @@ struct Engine final : EngineBase {
         cand_id.alloc(n * 4); pid.alloc(n * 4); mark.alloc(n * 4);
         alpha.alloc(n * rs); c.alloc(n * rs); T.alloc(n * rs); g.alloc(n * rs); rho.alloc(n * rs);
         E_el.alloc(n * rs);
+        rs_gain.alloc(n * rs); rs_best.alloc(n * 4); rs_valid.alloc(n); rs_dirty.alloc(n);
         g_comp.alloc(n * rs); work1.alloc(n * rs); work2.alloc(n * rs); work3.alloc(n * rs);
         work4.alloc(n * rs);
         bandA.alloc(n * 4); bandB.alloc(n * 4);
@@ -519,7 +525,10 @@ struct Engine final : EngineBase {
         if (!src || bytes != size_t(N) * elem(kind)) throw GpkError(GPK_E_INVALID, "upload size mismatch");
         HIPCHK(hipMemcpyAsync(b->p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               stream));
-        if (f == GPK_FIELD_PHI) check_labels();
+        if (f == GPK_FIELD_PHI) {
+            check_labels();
+            rs_reset = true;   // labels written from outside: no cached remap score holds
+        }
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -979,7 +988,8 @@ struct Engine final : EngineBase {
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
             phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
-            network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho());
+            network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho(), dims, dim3, remap_r,
+            rs_dirty.as<uint8_t>());
         return m;
     }
 
@@ -989,17 +999,26 @@ struct Engine final : EngineBase {
         const bool tiled = nlab <= 65536 && remap_r == 3;
         const int nb = band_compact(bandB, tiled);
         k_copy_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), phi_comp.as<int32_t>());
+        if (rs_reset || !tiled) {
+            fill<uint8_t>(rs_valid, N, 0);
+            fill<uint8_t>(rs_dirty, N, 0);
+            rs_reset = !tiled;
+        }
+        const ScoreCache<R> sc{rs_gain.as<R>(), rs_best.as<int32_t>(), rs_valid.as<uint8_t>(),
+                               rs_dirty.as<uint8_t>()};
         if (nb > 0 && tiled) {
             if (dim3) launch_small_tiled<3>([&](int ntx, int nty, int ntz, int grid) {
                 k_remap_score_tiled<R, 3, 3><<<grid, 256, 0, stream>>>(
                     dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), slot.as<int>(),
-                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>(), sc);
             });
             else launch_small_tiled<2>([&](int ntx, int nty, int ntz, int grid) {
                 k_remap_score_tiled<R, 2, 3><<<grid, 256, 0, stream>>>(
                     dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), slot.as<int>(),
-                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>());
+                    remapw.as<double>(), phi_comp.as<int32_t>(), cflag.as<uint8_t>(), gain_s.as<R>(), sc);
             });
+            k_score_cache_cleanup<<<gsblk(N, 256), 256, 0, stream>>>(N, rs_valid.as<uint8_t>(),
+                                                                     rs_dirty.as<uint8_t>());
         } else if (nb > 0) {
             if (dim3)
                 k_remap_score<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
@@ -1065,7 +1084,8 @@ struct Engine final : EngineBase {
                 k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
                     npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
                     Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds(),
-                    rho.as<R>(), R(model.rho_rex), sweep_rho());
+                    rho.as<R>(), R(model.rho_rex), sweep_rho(), dims, dim3, remap_r,
+                    rs_dirty.as<uint8_t>());
             }
         }
         band_compact(bandB, false);
@@ -1091,7 +1111,8 @@ struct Engine final : EngineBase {
             k_erode<R><<<1, 1024, 0, stream>>>(dims, dim3, n, ulist.as<int>(), unres.as<uint8_t>(),
                                                phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(),
                                                utgt.as<int>(), utgg.as<R>(), ds(), rho.as<R>(),
-                                               R(model.rho_rex), sweep_rho());
+                                               R(model.rho_rex), sweep_rho(), remap_r,
+                                               rs_dirty.as<uint8_t>());
         band_compact(bandB, false);
     }
 
@@ -1236,6 +1257,7 @@ struct Engine final : EngineBase {
     void voronoi(int n_seeds, uint32_t seed) override {
         if (n_seeds <= 0 || n_seeds > N)
             throw GpkError(GPK_E_INVALID, "Voronoi seed count must lie between one and the cell count");
+        rs_reset = true;
         Sites S = draw_sites(Nz, Ny, Nx, n_seeds, seed);
         std::vector<int32_t> h_phi(N);
         // nearest site on the device

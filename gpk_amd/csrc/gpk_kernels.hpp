@@ -451,12 +451,45 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
 
 // resident / best-candidate Gaussian scores of the balanced remap
 // (src/gpk_transitions.cpp:53-91) from LDS
+// Remap-score cache: a band cell's scores (gain, best label) depend only on
+// the labels of its (2r+1)^d window, so they are kept per cell and reused
+// until a label inside that window changes.  Every label write marks the
+// window-sized box around the written cell dirty; a scoring pass recomputes
+// dirty or unscored cells and a cleanup pass drops the cache of cells that
+// became dirty while off the band.
+template <class R>
+struct ScoreCache {
+    R* gain;
+    int32_t* best;
+    uint8_t* valid;     // null: no cache
+    uint8_t* dirty;
+};
+
+__device__ __forceinline__ void mark_dirty_box(const Dims& d, int dim3, int cell, int r, uint8_t* dirty) {
+    if (!dirty) return;
+    int z0, y0, x0;
+    decode(d, cell, z0, y0, x0);
+    const int rz = dim3 ? r : 0;
+    for (int dz = -rz; dz <= rz; ++dz) {
+        const int z = wrapi(z0 + dz, d.Nz);
+        for (int dy = -r; dy <= r; ++dy) {
+            const int y = wrapi(y0 + dy, d.Ny);
+            for (int ddx = -r; ddx <= r; ++ddx) dirty[lin(d, z, y, wrapi(x0 + ddx, d.Nx))] = 1;
+        }
+    }
+}
+
+__global__ void k_score_cache_cleanup(int n, uint8_t* __restrict__ valid, uint8_t* __restrict__ dirty) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (dirty[i]) { dirty[i] = 0; valid[i] = 0; }
+}
+
 template <class R, int DIM, int RS>
 __global__ void __launch_bounds__(256)
 k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
                     const int32_t* __restrict__ phi, const int* __restrict__ slot,
                     const double* __restrict__ wtab_g, int32_t* __restrict__ phi_comp,
-                    uint8_t* __restrict__ pflag, R* __restrict__ gain_s) {
+                    uint8_t* __restrict__ pflag, R* __restrict__ gain_s, ScoreCache<R> sc) {
     using H = HaloGeom<DIM, RS>;
     __shared__ uint16_t lab[H::NS];
     __shared__ double wt[3 * RS * RS + 1];
@@ -468,6 +501,29 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
     if (threadIdx.x == 0) nlist = 0;
+    __syncthreads();
+    // interface cells whose window is unchanged since their scores were
+    // computed take the cached scores; the others are listed for evaluation
+    for (int e = threadIdx.x; e < H::NT; e += blockDim.x) {
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
+        if (!(x < d.Nx && y < d.Ny && z < d.Nz)) continue;
+        const int i = lin(d, z, y, x);
+        if (!mask[i]) continue;
+        if (sc.valid && sc.valid[i] && !sc.dirty[i]) {
+            const int32_t bl = sc.best[i];
+            const R gain = sc.gain[i];
+            phi_comp[i] = bl;
+            const int a = slot[i];
+            pflag[a] = (phi[i] != bl && gain > Eps<R>::gain) ? 1 : 0;
+            gain_s[a] = gain;
+        } else {
+            list[atomicAdd(&nlist, 1)] = short(e);
+        }
+    }
+    __syncthreads();
+    const int n = nlist;
+    if (n == 0) return;
     for (int e = threadIdx.x; e < 3 * RS * RS + 1; e += blockDim.x) wt[e] = wtab_g[e];
     for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
         const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
@@ -475,13 +531,6 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
                                   wrapi(x0t + lx - RS, d.Nx))]);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < H::NT; e += blockDim.x) {
-        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
-        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
-        if (x < d.Nx && y < d.Ny && z < d.Nz && mask[lin(d, z, y, x)]) list[atomicAdd(&nlist, 1)] = short(e);
-    }
-    __syncthreads();
-    const int n = nlist;
     const int nface = DIM == 3 ? 6 : 4;
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const int e = list[q];
@@ -542,6 +591,12 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
         const int a = slot[i];
         pflag[a] = (p != bl && gain > Eps<R>::gain) ? 1 : 0;
         gain_s[a] = gain;
+        if (sc.valid) {
+            sc.best[i] = bl;
+            sc.gain[i] = gain;
+            sc.valid[i] = 1;
+            sc.dirty[i] = 0;
+        }
     }
 }
 
@@ -1090,7 +1145,8 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
                              const R* __restrict__ Canew, const int* __restrict__ state,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
-                             int all_accept, R* __restrict__ rho, R rho_rex, int sweep) {
+                             int all_accept, R* __restrict__ rho, R rho_rex, int sweep, Dims dd,
+                             int dim3, int rr, uint8_t* __restrict__ dirty) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool acc = false, def = false;
     if (k < m) {
@@ -1101,6 +1157,7 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
             g[cell] = Cg[k];
             alpha[cell] = R(1);
             if (sweep) rho[cell] = rho_rex;   // src/gpk_transitions.cpp:308
+            mark_dirty_box(dd, dim3, cell, rr, dirty);
             atomicSub(count + Cold[k], 1);
             atomicAdd(count + Cnew[k], 1);
             acc = true;
@@ -1372,7 +1429,7 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
                              const int* __restrict__ pstate, const R* __restrict__ Pg,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ pid, Scalars* s, R* __restrict__ rho, R rho_rex,
-                             int sweep) {
+                             int sweep, Dims dd, int dim3, int rr, uint8_t* __restrict__ dirty) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const int a = Pa[q], b = Pb[q];
@@ -1382,6 +1439,8 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
     g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
     alpha[a] = R(1); alpha[b] = R(1);
     if (sweep) { rho[a] = rho_rex; rho[b] = rho_rex; }   // src/gpk_transitions.cpp:173
+    mark_dirty_box(dd, dim3, a, rr, dirty);
+    mark_dirty_box(dd, dim3, b, rr, dirty);
     atomicAdd(&s->exchanges, 2);
 }
 
@@ -1411,7 +1470,7 @@ __global__ void __launch_bounds__(1024)
 k_erode(Dims d, int dim3, int n, const int* __restrict__ cells, const uint8_t* __restrict__ unres,
         int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
         int* __restrict__ count, int* __restrict__ tgt, R* __restrict__ tgg, Scalars* s,
-        R* __restrict__ rho, R rho_rex, int sweep) {
+        R* __restrict__ rho, R rho_rex, int sweep, int rr, uint8_t* __restrict__ dirty) {
     __shared__ int changed;
     const int nface = dim3 ? 6 : 4;
     while (true) {
@@ -1454,6 +1513,7 @@ k_erode(Dims d, int dim3, int n, const int* __restrict__ cells, const uint8_t* _
             g[i] = tgg[e];
             alpha[i] = R(1);
             if (sweep) rho[i] = rho_rex;   // src/gpk_transitions.cpp:245
+            mark_dirty_box(d, dim3, i, rr, dirty);
             atomicAdd(&changed, 1);
         }
         __syncthreads();
