@@ -68,17 +68,24 @@ __device__ __forceinline__ void row_sum(const LutRef<R>& lr, uint64_t m, R& R0, 
     row_sum<R>(lr.L, lr.t, m, R0, R1, R2);
 }
 
+// LawTag<1>: the model is known to be the plain isotropic law (kind
+// ISOTROPIC, eps4 = eps6 = 0, no elastic or dislocation term), for which the
+// general law gives dG = 0 and sigma = sigma0; the kernel is compiled without
+// the other laws.  LawTag<0>: the law is chosen from the descriptor at run time.
+template <int L> struct LawTag {};
+
 // competitor (src/gpk_geometry.cpp:154-182) and constitutive velocity
 // (src/gpk_physics.cpp:97-146) of band cell i (slot a) from its normal and
 // curvature
-template <class R, int DIM, class CubeAt>
+template <class R, int DIM, class CubeAt, int LAW = 0>
 __device__ __forceinline__ void comp_vel_cell(
     const Dims& d, const gpk_model_desc& model, int i, int a, int z0, int y0, int x0, int32_t res,
     R nxv, R nyv, R nzv, R kap, CubeAt cube_at, const int32_t* __restrict__ phi,
     const R* __restrict__ g, const R* __restrict__ c, const R* __restrict__ T,
     const R* __restrict__ rho, const R* __restrict__ E_el, R* __restrict__ vraw,
-    int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
+    int32_t* __restrict__ phi_comp, R* __restrict__ g_comp, LawTag<LAW> = LawTag<0>{}) {
     (void)cube_at;
+    constexpr bool plain = LAW == 1;
 
     // competitor: most-supported unlike face label, ties by outward-normal
     // alignment, then first face (src/gpk_geometry.cpp:154-182)
@@ -116,34 +123,37 @@ __device__ __forceinline__ void comp_vel_cell(
     int32_t qid = best >= 0 ? phi[f[best]] : res;
     // phase selection (src/gpk_physics.cpp:79-95): the lowest-G unlike face
     // label replaces the geometric competitor
-    if (model.kind == GPK_MODEL_PRECIPITATION) {
-        int32_t cand[6];
-        int nc = 0;
-        for (int k = 0; k < 6; ++k) {
-            const int32_t q = phi[f[k]];
-            if (q == res) continue;
-            bool seen = false;
-            for (int t = 0; t < nc; ++t) seen |= cand[t] == q;
-            if (!seen) cand[nc++] = q;
-        }
-        const int sel = law_select_phase<R>(model, res, c[i], T[i], cand, nc);
+    if (!plain && model.kind == GPK_MODEL_PRECIPITATION) {
+        int32_t fl[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) fl[k] = phi[f[k]];
+        const int sel = law_select_phase<R>(model, res, c[i], T[i], fl);
         if (sel >= 0) qid = sel;
     }
     phi_comp[i] = qid;
 
     // constitutive velocity v = M (dG - sigma(n) kappa) (src/gpk_physics.cpp:97-146)
     const R gpv = g[i];
-    R gq = gpv, cq = c[i];
+    R gq = gpv;
+    int fq = -1;
     for (int k = 0; k < 6; ++k)
-        if (phi[f[k]] == qid) { gq = g[f[k]]; cq = c[f[k]]; break; }
+        if (phi[f[k]] == qid) { fq = f[k]; break; }
+    if (fq >= 0) gq = g[fq];
     g_comp[i] = gq;
-    const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
-    R dG = law_driving_force<R>(model, res, qid, ci, T[i]);
-    const R sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
+    R dG, sig;
+    if (plain) {
+        dG = R(0);
+        sig = R(model.sigma0);
+    } else {
+        const R cq = fq >= 0 ? c[fq] : c[i];
+        const R ci = law_interface_composition<R>(model, res, qid, c[i], cq);
+        dG = law_driving_force<R>(model, res, qid, ci, T[i]);
+        sig = law_energy<R>(model, nxv, nyv, nzv, gpv);
+    }
     const R mob = R(model.M0);
     // elastic energy of the product side minus the parent's
     // (src/gpk_physics.cpp:122-127)
-    if (model.flags & GPK_FLAG_ELASTIC) {
+    if (!plain && (model.flags & GPK_FLAG_ELASTIC)) {
         R ep = E_el[i];
         for (int k = 0; k < 6; ++k)
             if (phi[f[k]] == qid) { ep = E_el[f[k]]; break; }
@@ -151,7 +161,7 @@ __device__ __forceinline__ void comp_vel_cell(
     }
     // stored dislocation energy of the product side minus the parent's
     // (src/gpk_physics.cpp:131-140)
-    if (model.flags & GPK_FLAG_DISLOCATION) {
+    if (!plain && (model.flags & GPK_FLAG_DISLOCATION)) {
         R rp = rho[i];
         for (int k = 0; k < 6; ++k)
             if (phi[f[k]] == qid) { rp = rho[f[k]]; break; }
@@ -162,7 +172,7 @@ __device__ __forceinline__ void comp_vel_cell(
 
 // normals, curvature (src/gpk_geometry.cpp:136-151), competitor (154-182) and
 // the constitutive velocity of band cell i (slot a) from its stencil sums
-template <class R, int DIM, class CubeAt>
+template <class R, int DIM, class CubeAt, int LAW = 0>
 __device__ __forceinline__ void finish_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int i, int a, int z0, int y0,
     int x0, int32_t res, R Ix, R Iy, R Iz, R Ixx, R Iyy, R Izz, R Ixy, R Ixz, R Iyz, CubeAt cube_at,
@@ -171,7 +181,7 @@ __device__ __forceinline__ void finish_cell(
     R* __restrict__ ony,
     R* __restrict__ onz,
     R* __restrict__ okappa, R* __restrict__ vraw, int32_t* __restrict__ phi_comp,
-    R* __restrict__ g_comp) {
+    R* __restrict__ g_comp, LawTag<LAW> law = LawTag<0>{}) {
     const R dx = R(gp.dx_f);
     const R inv_dx = R(1) / dx;
     const R inv_dx2 = inv_dx * inv_dx;
@@ -193,7 +203,7 @@ __device__ __forceinline__ void finish_cell(
     }
     onx[a] = nxv; ony[a] = nyv; onz[a] = nzv; okappa[a] = kap;
     comp_vel_cell<R, DIM>(d, model, i, a, z0, y0, x0, res, nxv, nyv, nzv, kap, cube_at, phi, g, c, T,
-                          rho, E_el, vraw, phi_comp, g_comp);
+                          rho, E_el, vraw, phi_comp, g_comp, law);
 }
 
 

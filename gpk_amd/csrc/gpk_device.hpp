@@ -147,14 +147,19 @@ __device__ __forceinline__ R law_gibbs(const gpk_model_desc& d, int q, R cv, R T
 }
 
 // PrecipitationThermo::select_product_phase (include/gpk_thermo.hpp:371-383)
+// over the six face labels in face order.  The reference's candidate list is
+// the distinct unlike face labels in first-occurrence order; a repeated label
+// has the same dG and never wins the strict "<" a second time, and the parent
+// label is skipped, so scanning the faces directly gives the same answer.
 template <class R>
 __device__ __forceinline__ int law_select_phase(const gpk_model_desc& d, int32_t pp, R cv, R Tv,
-                                                const int32_t* cand, int nc) {
+                                                const int32_t (&fl)[6]) {
     R best = R(1e30f);
     int best_q = -1;
     if (pp < 0 || pp >= d.n_phases) return -1;
-    for (int k = 0; k < nc; ++k) {
-        const int32_t q = cand[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int32_t q = fl[k];
         if (q < 0 || q >= d.n_phases || q == pp) continue;
         const R dG = law_gibbs<R>(d, q, cv, Tv) - law_gibbs<R>(d, pp, cv, Tv);
         if (dG < best) { best = dG; best_q = q; }

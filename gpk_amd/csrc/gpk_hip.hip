@@ -619,7 +619,11 @@ struct Engine final : EngineBase {
                   ntz = (Nz + S::TZ - 1) / S::TZ;
         const int ntiles = ntx * nty * ntz;
         const int grid = 8 * ((ntiles + 7) / 8);
-        k_geometry_tiled<R, DIM, RG><<<grid, 256, 0, stream>>>(
+        const bool plain_iso = model.kind == GPK_MODEL_ISOTROPIC && model.eps4 == 0.f &&
+                               model.eps6 == 0.f &&
+                               !(model.flags & (GPK_FLAG_ELASTIC | GPK_FLAG_DISLOCATION));
+        auto kern = plain_iso ? k_geometry_tiled<R, DIM, RG, 1> : k_geometry_tiled<R, DIM, RG, 0>;
+        kern<<<grid, 256, 0, stream>>>(
             dims, gp, model, ntx, nty, ntz, phi.as<int32_t>(), mask.as<uint8_t>(), slot.as<int>(),
             wt0.as<R>(), wt1.as<R>(), wt2.as<R>(), tab.as<R>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(),
             nx.as<R>(), ny.as<R>(), nz.as<R>(), kappa.as<R>(), vraw.as<R>(), phi_comp.as<int32_t>(),

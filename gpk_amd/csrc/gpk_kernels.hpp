@@ -74,7 +74,7 @@ template <int DIM> struct TileShape;
 template <> struct TileShape<3> { static constexpr int TZ = 8, TY = 8, TX = 32; };
 template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 64, TX = 32; };
 
-constexpr int GEO_MAXRES = 8;    // residents per mask batch
+constexpr int GEO_MAXRES = 4;    // residents per mask batch
 constexpr int GEO_HCAP = 256;    // resident hash slots per tile
 constexpr int GEO_MAXB = GEO_HCAP / GEO_MAXRES;   // batches per tile
 
@@ -101,7 +101,7 @@ struct TiledLds {
 
 // stencil sums of one interface cell from its window row masks
 // (rowmask(jz, jy): bit jx set when window position jx carries the resident)
-template <class R, int DIM, int RG, class RowMask>
+template <class R, int DIM, int RG, int LAW, class RowMask>
 __device__ __forceinline__ void tiled_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0,
     int32_t p, const R* lut, const R (*wy)[2 * RG + 1], RowMask rowmask,
@@ -152,10 +152,12 @@ __device__ __forceinline__ void tiled_cell(
         return phi[lin(d, zz, yy, xx)];
     };
     finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, p, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz,
-                        Iyz, cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
+                        Iyz, cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp,
+                        LawTag<LAW>{});
 }
 
-template <class R, int DIM, int RG>
+// LAW = 1: compiled for the plain isotropic law only (LawTag<1>).
+template <class R, int DIM, int RG, int LAW>
 __global__ void __launch_bounds__(256)
 k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, int ntz,
                  const int32_t* __restrict__ phi, const uint8_t* __restrict__ mask,
@@ -169,12 +171,17 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
     constexpr int NT = S::TZ * S::TY * S::TX;
-    __shared__ unsigned long long rm[S::NROWS * GEO_MAXRES];
+    static_assert(NT <= 4096, "tile index must fit 12 bits");
+    // rm holds the row masks of one resident batch; before the first batch the
+    // same memory holds the tile's interface list and each listed cell's
+    // resident index (2 + 2 bytes per cell)
+    constexpr int NRM = S::NROWS * GEO_MAXRES > NT / 2 ? S::NROWS * GEO_MAXRES : NT / 2;
+    __shared__ unsigned long long rm[NRM];
+    short* const list = reinterpret_cast<short*>(rm);
+    unsigned short* const cres = reinterpret_cast<unsigned short*>(rm) + NT;
     __shared__ R lut[S::NLUT];
     __shared__ R wy[3][S::W];
-    __shared__ short list[NT];
-    __shared__ unsigned short cres[NT];
-    __shared__ short blist[NT];           // cells grouped by resident batch
+    __shared__ uint32_t blist[NT];        // (resident index << 16) | tile cell, grouped by batch
     __shared__ int hkey[GEO_HCAP];
     __shared__ short hidx[GEO_HCAP];
     __shared__ int resl[GEO_HCAP];
@@ -245,36 +252,50 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     }
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int b = cres[q] == 0xFFFFu ? GEO_MAXB : cres[q] / GEO_MAXRES;
-        blist[boff[b] + atomicAdd(&bcount[b], 1)] = short(q);
+        const uint32_t r = cres[q];
+        const int b = r == 0xFFFFu ? GEO_MAXB : int(r) / GEO_MAXRES;
+        blist[boff[b] + atomicAdd(&bcount[b], 1)] = (r << 16) | uint32_t(uint16_t(list[q]));
     }
     const int nr = nres;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    constexpr int INFL = 4;               // staged rows in flight per wavefront
     for (int b0 = 0; b0 < nr; b0 += GEO_MAXRES) {
         const int nbat = nr - b0 < GEO_MAXRES ? nr - b0 : GEO_MAXRES;
         const int bb = b0 / GEO_MAXRES;
         __syncthreads();
         // row masks: wave per staged row, one ballot per resident of the batch
-        for (int r = wave; r < S::NROWS; r += nwave) {
-            const int lz = r / S::SY, ly = r % S::SY;
-            int v = -1;
-            if (lane < S::SX)
-                v = phi[lin(d, wrapi(z0t + lz - S::HZ, d.Nz), wrapi(y0t + ly - RG, d.Ny),
-                            wrapi(x0t + lane - RG, d.Nx))];
-            for (int t = 0; t < nbat; ++t) {
-                const unsigned long long m = __ballot(v == resl[b0 + t]);
-                if (lane == 0) rm[r * GEO_MAXRES + t] = m;
+        for (int r0 = wave; r0 < S::NROWS; r0 += INFL * nwave) {
+            int v[INFL];
+#pragma unroll
+            for (int k = 0; k < INFL; ++k) {
+                const int r = r0 + k * nwave;
+                v[k] = -1;
+                if (r < S::NROWS && lane < S::SX) {
+                    const int lz = r / S::SY, ly = r % S::SY;
+                    v[k] = phi[lin(d, wrapi(z0t + lz - S::HZ, d.Nz), wrapi(y0t + ly - RG, d.Ny),
+                                   wrapi(x0t + lane - RG, d.Nx))];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < INFL; ++k) {
+                const int r = r0 + k * nwave;
+                if (r < S::NROWS) {
+                    for (int t = 0; t < nbat; ++t) {
+                        const unsigned long long msk = __ballot(v[k] == resl[b0 + t]);
+                        if (lane == 0) rm[r * GEO_MAXRES + t] = msk;
+                    }
+                }
             }
         }
         __syncthreads();
         const int bend = boff[bb] + bcount[bb];
         for (int u = boff[bb] + threadIdx.x; u < bend; u += blockDim.x) {
-            const int q = blist[u];
-            const int rs = int(cres[q]) - b0;
-            const int e = list[q];
+            const uint32_t pk = blist[u];
+            const int e = int(pk & 0xFFFFu);
+            const int r = int(pk >> 16);
             const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
-            const unsigned long long* rows = rm + (lz * S::SY + ly) * GEO_MAXRES + rs;
-            tiled_cell<R, DIM, RG>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[cres[q]], lut, wy,
+            const unsigned long long* rows = rm + (lz * S::SY + ly) * GEO_MAXRES + (r - b0);
+            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[r], lut, wy,
                                    [&](int jz, int jy) {
                                        return rows[(jz * S::SY + jy) * GEO_MAXRES] >> lx;
                                    },
@@ -284,20 +305,19 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     // residents that did not fit the hash: window rows read from the grid
     __syncthreads();
     for (int u = boff[GEO_MAXB] + threadIdx.x; u < boff[GEO_MAXB] + bcount[GEO_MAXB]; u += blockDim.x) {
-        const int q = blist[u];
-        const int e = list[q];
+        const int e = int(blist[u] & 0xFFFFu);
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
         const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
         const int32_t p = phi[lin(d, z0, y0, x0)];
-        tiled_cell<R, DIM, RG>(d, gp, model, x0, y0, z0, p, lut, wy,
+        tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0, y0, z0, p, lut, wy,
                                [&](int jz, int jy) {
                                    const int z = DIM == 3 ? wrapi(z0 + jz - RG, d.Nz) : z0;
                                    const int y = wrapi(y0 + jy - RG, d.Ny);
-                                   unsigned long long m = 0;
+                                   unsigned long long mm = 0;
                                    for (int jx = 0; jx < S::W; ++jx)
                                        if (phi[lin(d, z, y, wrapi(x0 + jx - RG, d.Nx))] == p)
-                                           m |= 1ull << jx;
-                                   return m;
+                                           mm |= 1ull << jx;
+                                   return mm;
                                },
                                phi, slot, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
     }
