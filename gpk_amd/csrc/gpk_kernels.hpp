@@ -74,12 +74,14 @@ template <int DIM> struct TileShape;
 template <> struct TileShape<3> { static constexpr int TZ = 8, TY = 8, TX = 32; };
 template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 64, TX = 32; };
 
-constexpr int GEO_MAXRES = 4;    // residents per mask batch
 constexpr int GEO_HCAP = 256;    // resident hash slots per tile
-constexpr int GEO_MAXB = GEO_HCAP / GEO_MAXRES;   // batches per tile
 
 template <class R, int DIM, int RG>
 struct TiledLds {
+    // residents per mask batch: 3 in FP64 and 4 in FP32 keep a 256-thread
+    // workgroup's LDS within a quarter of the CU's 160 KB at radius 8
+    static constexpr int MAXRES = sizeof(R) == 8 ? 3 : 4;
+    static constexpr int MAXB = (GEO_HCAP + MAXRES - 1) / MAXRES;   // batches per tile
     static constexpr int TZ = TileShape<DIM>::TZ, TY = TileShape<DIM>::TY, TX = TileShape<DIM>::TX;
     static constexpr int HZ = DIM == 3 ? RG : 0;
     static constexpr int SZ = TZ + 2 * HZ, SY = TY + 2 * RG, SX = TX + 2 * RG;
@@ -156,9 +158,12 @@ __device__ __forceinline__ void tiled_cell(
                         LawTag<LAW>{});
 }
 
-// LAW = 1: compiled for the plain isotropic law only (LawTag<1>).
+// LAW = 1: compiled for the plain isotropic law only (LawTag<1>).  At radius 8
+// (sigma 1.5) that instance fits 128 VGPRs and 40 KB of LDS, i.e. four
+// workgroups per CU; the register bound is asked of the compiler only there,
+// since other radii would spill.
 template <class R, int DIM, int RG, int LAW>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, LAW == 1 && RG == 8 ? 4 : 1)
 k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, int ntz,
                  const int32_t* __restrict__ phi, const uint8_t* __restrict__ mask,
                  const int* __restrict__ slot, const R* __restrict__ w0,
@@ -171,21 +176,21 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
     constexpr int NT = S::TZ * S::TY * S::TX;
-    static_assert(NT <= 4096, "tile index must fit 12 bits");
+    static_assert(NT <= 4096 && S::MAXRES <= 16, "tile index and batch slot must fit 16 bits");
     // rm holds the row masks of one resident batch; before the first batch the
     // same memory holds the tile's interface list and each listed cell's
     // resident index (2 + 2 bytes per cell)
-    constexpr int NRM = S::NROWS * GEO_MAXRES > NT / 2 ? S::NROWS * GEO_MAXRES : NT / 2;
+    constexpr int NRM = S::NROWS * S::MAXRES > NT / 2 ? S::NROWS * S::MAXRES : NT / 2;
     __shared__ unsigned long long rm[NRM];
     short* const list = reinterpret_cast<short*>(rm);
     unsigned short* const cres = reinterpret_cast<unsigned short*>(rm) + NT;
     __shared__ R lut[S::NLUT];
     __shared__ R wy[3][S::W];
-    __shared__ uint32_t blist[NT];        // (resident index << 16) | tile cell, grouped by batch
+    __shared__ uint16_t blist[NT];        // (slot in the batch << 12) | tile cell, grouped by batch
     __shared__ int hkey[GEO_HCAP];
     __shared__ short hidx[GEO_HCAP];
     __shared__ int resl[GEO_HCAP];
-    __shared__ int bcount[GEO_MAXB + 1], boff[GEO_MAXB + 1];
+    __shared__ int bcount[S::MAXB + 1], boff[S::MAXB + 1];
     __shared__ int nlist, nres;
 
     const int ntiles = ntx * nty * ntz;
@@ -197,7 +202,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
 
     if (threadIdx.x == 0) { nlist = 0; nres = 0; }
     for (int e = threadIdx.x; e < GEO_HCAP; e += blockDim.x) hkey[e] = -1;
-    for (int e = threadIdx.x; e <= GEO_MAXB; e += blockDim.x) bcount[e] = 0;
+    for (int e = threadIdx.x; e <= S::MAXB; e += blockDim.x) bcount[e] = 0;
     for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e] = lut_g[e];
     for (int e = threadIdx.x; e < S::W; e += blockDim.x) {
         wy[0][e] = w0[e]; wy[1][e] = w1[e]; wy[2][e] = w2[e];
@@ -243,25 +248,26 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
             h = (h + 1) & (GEO_HCAP - 1);
         }
         cres[q] = r;
-        atomicAdd(&bcount[r == 0xFFFFu ? GEO_MAXB : r / GEO_MAXRES], 1);
+        atomicAdd(&bcount[r == 0xFFFFu ? S::MAXB : r / S::MAXRES], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int o = 0;
-        for (int b = 0; b <= GEO_MAXB; ++b) { boff[b] = o; o += bcount[b]; bcount[b] = 0; }
+        for (int b = 0; b <= S::MAXB; ++b) { boff[b] = o; o += bcount[b]; bcount[b] = 0; }
     }
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
         const uint32_t r = cres[q];
-        const int b = r == 0xFFFFu ? GEO_MAXB : int(r) / GEO_MAXRES;
-        blist[boff[b] + atomicAdd(&bcount[b], 1)] = (r << 16) | uint32_t(uint16_t(list[q]));
+        const int b = r == 0xFFFFu ? S::MAXB : int(r) / S::MAXRES;
+        const uint32_t t = r == 0xFFFFu ? 0u : r % S::MAXRES;
+        blist[boff[b] + atomicAdd(&bcount[b], 1)] = uint16_t((t << 12) | uint32_t(list[q]));
     }
     const int nr = nres;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
     constexpr int INFL = 4;               // staged rows in flight per wavefront
-    for (int b0 = 0; b0 < nr; b0 += GEO_MAXRES) {
-        const int nbat = nr - b0 < GEO_MAXRES ? nr - b0 : GEO_MAXRES;
-        const int bb = b0 / GEO_MAXRES;
+    for (int b0 = 0; b0 < nr; b0 += S::MAXRES) {
+        const int nbat = nr - b0 < S::MAXRES ? nr - b0 : S::MAXRES;
+        const int bb = b0 / S::MAXRES;
         __syncthreads();
         // row masks: wave per staged row, one ballot per resident of the batch
         for (int r0 = wave; r0 < S::NROWS; r0 += INFL * nwave) {
@@ -282,7 +288,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                 if (r < S::NROWS) {
                     for (int t = 0; t < nbat; ++t) {
                         const unsigned long long msk = __ballot(v[k] == resl[b0 + t]);
-                        if (lane == 0) rm[r * GEO_MAXRES + t] = msk;
+                        if (lane == 0) rm[r * S::MAXRES + t] = msk;
                     }
                 }
             }
@@ -291,21 +297,21 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         const int bend = boff[bb] + bcount[bb];
         for (int u = boff[bb] + threadIdx.x; u < bend; u += blockDim.x) {
             const uint32_t pk = blist[u];
-            const int e = int(pk & 0xFFFFu);
-            const int r = int(pk >> 16);
+            const int e = int(pk & 0xFFFu);
+            const int t = int(pk >> 12);
             const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
-            const unsigned long long* rows = rm + (lz * S::SY + ly) * GEO_MAXRES + (r - b0);
-            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[r], lut, wy,
+            const unsigned long long* rows = rm + (lz * S::SY + ly) * S::MAXRES + t;
+            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[b0 + t], lut, wy,
                                    [&](int jz, int jy) {
-                                       return rows[(jz * S::SY + jy) * GEO_MAXRES] >> lx;
+                                       return rows[(jz * S::SY + jy) * S::MAXRES] >> lx;
                                    },
                                    phi, slot, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
         }
     }
     // residents that did not fit the hash: window rows read from the grid
     __syncthreads();
-    for (int u = boff[GEO_MAXB] + threadIdx.x; u < boff[GEO_MAXB] + bcount[GEO_MAXB]; u += blockDim.x) {
-        const int e = int(blist[u] & 0xFFFFu);
+    for (int u = boff[S::MAXB] + threadIdx.x; u < boff[S::MAXB] + bcount[S::MAXB]; u += blockDim.x) {
+        const int e = int(blist[u] & 0xFFFu);
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
         const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
         const int32_t p = phi[lin(d, z0, y0, x0)];
