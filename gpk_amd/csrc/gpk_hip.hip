@@ -759,13 +759,13 @@ struct Engine final : EngineBase {
         // the tiled projection packs (low, high, orientation) in 32 bits: labels < 2^15
         if (nb > 0 && nlab <= 32768 && proj_r == 3) {
             if (dim3) launch_small_tiled<3>([&](int ntx, int nty, int ntz, int grid) {
-                k_project_tiled<R, 3, 3><<<grid, 256, 0, stream>>>(
+                k_project_tiled<R, 3, 3><<<grid, PROJ_THREADS, 0, stream>>>(
                     dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), phi_comp.as<int32_t>(),
                     slot.as<int>(), projw.as<double>(), vraw.as<R>(), vn.as<R>(),
                     reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
             });
             else launch_small_tiled<2>([&](int ntx, int nty, int ntz, int grid) {
-                k_project_tiled<R, 2, 3><<<grid, 256, 0, stream>>>(
+                k_project_tiled<R, 2, 3><<<grid, PROJ_THREADS, 0, stream>>>(
                     dims, ntx, nty, ntz, mask.as<uint8_t>(), phi.as<int32_t>(), phi_comp.as<int32_t>(),
                     slot.as<int>(), projw.as<double>(), vraw.as<R>(), vn.as<R>(),
                     reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));

@@ -388,18 +388,26 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
     return (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
 }
 
+// +0.0 unless the predicate holds (bit mask, no branch)
+__device__ __forceinline__ double keep_if(bool hit, double v) {
+    const unsigned long long m = hit ? ~0ull : 0ull;
+    return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(v) & m));
+}
+
 // same-pair projection of v (src/gpk_physics.cpp:152-193) from LDS
+constexpr int PROJ_THREADS = 512;   // 16 wavefronts per CU at two tiles' LDS
+
 template <class R, int DIM, int RP>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(PROJ_THREADS)
 k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
                 const int32_t* __restrict__ phi, const int32_t* __restrict__ phi_comp,
                 const int* __restrict__ slot, const double* __restrict__ wtab_g,
                 const R* __restrict__ vraw, R* __restrict__ vout,
                 typename UBits<R>::T* __restrict__ vmax_bits) {
     using H = HaloGeom<DIM, RP>;
+    constexpr int W = 2 * RP + 1;
     __shared__ uint32_t key[H::NS];
     __shared__ R vv[H::NS];
-    __shared__ double wt[3 * RP * RP + 1];
     __shared__ short list[H::NT];
     __shared__ int nlist;
     const int ntiles = ntx * nty * ntz;
@@ -408,26 +416,47 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
     if (threadIdx.x == 0) nlist = 0;
-    for (int e = threadIdx.x; e < 3 * RP * RP + 1; e += blockDim.x) wt[e] = wtab_g[e];
     // key = ((low << 16 | high) << 1 | [resident is the high label]) for
     // interface cells whose two labels differ, all ones otherwise (labels
-    // below 2^15, so the packed pair fits 31 bits and never equals all ones)
-    for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
-        const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
-        const int j = lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RP, d.Ny),
-                          wrapi(x0t + lx - RP, d.Nx));
-        uint32_t k = 0xFFFFFFFFu;
-        R v = R(0);
-        if (mask[j]) {
-            const uint32_t p = uint32_t(phi[j]), q = uint32_t(phi_comp[j]);
-            if (p != q) {
-                const uint32_t lo = p < q ? p : q, hi = p < q ? q : p;
-                k = (((lo << 16) | hi) << 1) | (p == lo ? 0u : 1u);
-            }
-            v = vraw[slot[j]];
+    // below 2^15, so the packed pair fits 31 bits and never equals all ones).
+    // Four staged cells per thread are loaded at once: mask, labels and band
+    // slot together, then the velocity of the interface ones.
+    constexpr int INFL = 4;
+    for (int e0 = threadIdx.x; e0 < H::NS; e0 += INFL * blockDim.x) {
+        int j[INFL], sl[INFL];
+        uint32_t p[INFL], q[INFL];
+        uint8_t mk[INFL];
+#pragma unroll
+        for (int k = 0; k < INFL; ++k) {
+            const int e = e0 + k * blockDim.x;
+            const int ee = e < H::NS ? e : 0;
+            const int lx = ee % H::SX, ly = (ee / H::SX) % H::SY, lz = ee / (H::SX * H::SY);
+            j[k] = lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RP, d.Ny),
+                       wrapi(x0t + lx - RP, d.Nx));
         }
-        key[e] = k;
-        vv[e] = v;
+#pragma unroll
+        for (int k = 0; k < INFL; ++k) {
+            mk[k] = mask[j[k]];
+            p[k] = uint32_t(phi[j[k]]);
+            q[k] = uint32_t(phi_comp[j[k]]);
+            sl[k] = slot[j[k]];
+        }
+        R v[INFL];
+#pragma unroll
+        for (int k = 0; k < INFL; ++k) v[k] = mk[k] ? vraw[sl[k]] : R(0);
+#pragma unroll
+        for (int k = 0; k < INFL; ++k) {
+            const int e = e0 + k * blockDim.x;
+            if (e < H::NS) {
+                uint32_t kk = 0xFFFFFFFFu;
+                if (mk[k] && p[k] != q[k]) {
+                    const uint32_t lo = p[k] < q[k] ? p[k] : q[k], hi = p[k] < q[k] ? q[k] : p[k];
+                    kk = (((lo << 16) | hi) << 1) | (p[k] == lo ? 0u : 1u);
+                }
+                key[e] = kk;
+                vv[e] = v[k];
+            }
+        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < H::NT; e += blockDim.x) {
@@ -437,8 +466,8 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     }
     __syncthreads();
     const int n = nlist;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int e = list[q];
+    for (int qq = threadIdx.x; qq < n; qq += blockDim.x) {
+        const int e = list[qq];
         const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
         const int c = ((lz + H::HZ) * H::SY + (ly + RP)) * H::SX + (lx + RP);
         const uint32_t ki = key[c];
@@ -447,23 +476,32 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
             const uint32_t pair = ki >> 1;
             const R oi = (ki & 1u) ? R(-1) : R(1);
             double num = 0.0, den = 0.0;
-            const int nzw = DIM == 3 ? 2 * RP + 1 : 1;
+            const int nzw = DIM == 3 ? W : 1;
             for (int jz = 0; jz < nzw; ++jz) {
                 const int dz = DIM == 3 ? jz - RP : 0;
-                for (int jy = 0; jy <= 2 * RP; ++jy) {
+                for (int jy = 0; jy < W; ++jy) {
                     const int dy = jy - RP;
                     const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RP + dy)) * H::SX + lx;
-                    const double* wr = wt + dy * dy + dz * dz;
+                    // the row's labels and velocities first, then the sums in
+                    // window order; a miss adds +0.0, which leaves the sums
+                    // unchanged (they never hold -0)
+                    uint32_t kj[W];
+                    R vj[W];
 #pragma unroll
-                    for (int jx = 0; jx <= 2 * RP; ++jx) {
+                    for (int jx = 0; jx < W; ++jx) { kj[jx] = key[rowb + jx]; vj[jx] = vv[rowb + jx]; }
+                    const double* wr = wtab_g + dy * dy + dz * dz;
+#pragma unroll
+                    for (int jx = 0; jx < W; ++jx) {
                         const int dxx = jx - RP;
-                        const uint32_t kj = key[rowb + jx];
-                        const bool hit = (kj >> 1) == pair;
+                        const bool hit = (kj[jx] >> 1) == pair;
                         const double w = wr[dxx * dxx];
-                        const double wo = (kj & 1u) ? -w : w;
-                        // adding +0.0 leaves the sums unchanged (they never hold -0)
-                        num += hit ? wo * double(vv[rowb + jx]) : 0.0;
-                        den += hit ? w : 0.0;
+                        // (-w) v == -(w v) exactly: the orientation flips the
+                        // product's sign bit
+                        const double t = __longlong_as_double(
+                            __double_as_longlong(w * double(vj[jx])) ^
+                            (static_cast<long long>(kj[jx] & 1u) << 63));
+                        num += keep_if(hit, t);
+                        den += keep_if(hit, w);
                     }
                 }
             }
