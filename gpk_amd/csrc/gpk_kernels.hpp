@@ -548,15 +548,47 @@ __global__ void k_score_cache_cleanup(int n, uint8_t* __restrict__ valid, uint8_
         if (dirty[i]) { dirty[i] = 0; valid[i] = 0; }
 }
 
+// Gaussian label scores over one cell's (2RS+1)^d window, read from the
+// staged labels: acc[0] sums the weights of the resident label p, acc[1 + t]
+// those of candidate label L[1 + t] for t < NC, each in window order.  NC is
+// the wavefront's largest candidate count, so every lane walks the same
+// accumulators; a lane with fewer candidates holds -1 there, which matches no
+// label.
+template <int NC, int DIM, int RS, class H>
+__device__ __forceinline__ void remap_window(const uint16_t* lab, const double* __restrict__ wtab_g,
+                                             int lz, int ly, int lx, int32_t p, const int32_t (&L)[7],
+                                             double (&acc)[7]) {
+    constexpr int W = 2 * RS + 1;
+    const int nzw = DIM == 3 ? W : 1;
+    for (int jz = 0; jz < nzw; ++jz) {
+        const int dz = DIM == 3 ? jz - RS : 0;
+        for (int jy = 0; jy < W; ++jy) {
+            const int dy = jy - RS;
+            const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RS + dy)) * H::SX + lx;
+            const double* wr = wtab_g + dy * dy + dz * dz;
+            int32_t v[W];
+#pragma unroll
+            for (int jx = 0; jx < W; ++jx) v[jx] = lab[rowb + jx];
+#pragma unroll
+            for (int jx = 0; jx < W; ++jx) {
+                const int dxx = jx - RS;
+                const double w = wr[dxx * dxx];
+                acc[0] += keep_if(v[jx] == p, w);
+#pragma unroll
+                for (int t = 0; t < NC; ++t) acc[1 + t] += keep_if(v[jx] == L[1 + t], w);
+            }
+        }
+    }
+}
+
 template <class R, int DIM, int RS>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, 6)
 k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
                     const int32_t* __restrict__ phi, const int* __restrict__ slot,
                     const double* __restrict__ wtab_g, int32_t* __restrict__ phi_comp,
                     uint8_t* __restrict__ pflag, R* __restrict__ gain_s, ScoreCache<R> sc) {
     using H = HaloGeom<DIM, RS>;
     __shared__ uint16_t lab[H::NS];
-    __shared__ double wt[3 * RS * RS + 1];
     __shared__ short list[H::NT];
     __shared__ int nlist;
     const int ntiles = ntx * nty * ntz;
@@ -588,7 +620,6 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
     __syncthreads();
     const int n = nlist;
     if (n == 0) return;
-    for (int e = threadIdx.x; e < 3 * RS * RS + 1; e += blockDim.x) wt[e] = wtab_g[e];
     for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
         const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
         lab[e] = uint16_t(phi[lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RS, d.Ny),
@@ -623,24 +654,14 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
             ncw = o > ncw ? o : ncw;
         }
         double acc[7] = {0, 0, 0, 0, 0, 0, 0};
-        const int nzw = DIM == 3 ? 2 * RS + 1 : 1;
-        for (int jz = 0; jz < nzw; ++jz) {
-            const int dz = DIM == 3 ? jz - RS : 0;
-            for (int jy = 0; jy <= 2 * RS; ++jy) {
-                const int dy = jy - RS;
-                const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RS + dy)) * H::SX + lx;
-                const double* wr = wt + dy * dy + dz * dz;
-#pragma unroll
-                for (int jx = 0; jx <= 2 * RS; ++jx) {
-                    const int dxx = jx - RS;
-                    const int32_t v = lab[rowb + jx];
-                    const double w = wr[dxx * dxx];
-                    acc[0] += v == p ? w : 0.0;
-#pragma unroll
-                    for (int t = 0; t < 6; ++t)
-                        if (t < ncw) acc[1 + t] += v == L[1 + t] ? w : 0.0;
-                }
-            }
+        switch (ncw) {     // wavefront-uniform
+        case 0: remap_window<0, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        case 1: remap_window<1, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        case 2: remap_window<2, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        case 3: remap_window<3, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        case 4: remap_window<4, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        case 5: remap_window<5, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
+        default: remap_window<6, DIM, RS, H>(lab, wtab_g, lz, ly, lx, p, L, acc); break;
         }
         const double resident = acc[0];
         double sup[6];
