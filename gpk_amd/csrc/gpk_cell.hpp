@@ -377,6 +377,8 @@ __device__ __forceinline__ R remap_score_cell(const Dims& d, int r, int i,
     return eb - es;
 }
 
+__device__ __forceinline__ bool bounded_cert_masks(bool d3, Bits128 bo, Bits128 bn, int old_count);
+
 // bounded (5^d box) certificate for relabelling `cell` from ol to nl, labels
 // read through label_at (src/gpk_connectivity.cpp:10-195 with local_only)
 template <class LabelAt>
@@ -400,6 +402,12 @@ __device__ bool bounded_cert(const Dims& d, bool d3, int cell, int32_t ol, int32
             }
         }
     }
+    return bounded_cert_masks(d3, bo, bn, old_count);
+}
+
+// the certificate's decision from the box masks of the old (bo) and new (bn)
+// label, bit lz * 25 + ly * 5 + lx
+__device__ __forceinline__ bool bounded_cert_masks(bool d3, Bits128 bo, Bits128 bn, int old_count) {
     const int lc = (d3 ? 2 : 0) * 25 + 12;
     // 3^d cube occupancy from the box
     uint32_t co = 0, cn = 0;

@@ -269,7 +269,7 @@ struct Engine final : EngineBase {
     DevBuf Pk, Ps;
     DevBuf gain_s, pcell, kgain, kgain2, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
         pflag2, pidx;
-    DevBuf Pa, Pb, Plo, Phi, pstate, Pg;
+    DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps;
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
@@ -337,7 +337,7 @@ struct Engine final : EngineBase {
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
                                &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain, &kgain2,
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
-                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &count, &unres, &ulist,
+                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
                                &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
                                &rs_gain, &rs_best, &rs_valid, &rs_dirty,
@@ -1076,10 +1076,12 @@ struct Engine final : EngineBase {
                 pc.Pa = Pa.as<int>(); pc.Pb = Pb.as<int>(); pc.Plo = Plo.as<int>(); pc.Phi = Phi.as<int>();
                 pc.pstate = pstate.as<int>(); pc.pid = pid.as<int>(); pc.phi = phi.as<int32_t>();
                 pc.g = g.p; pc.Pg = Pg.p; pc.count = count.as<int>(); pc.s = ds();
+                if (pdeps.bytes < size_t(npairs) * PAIR_DEPS * 4) pdeps.alloc(size_t(npairs) * PAIR_DEPS * 4);
+                k_pair_deps<<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>());
                 int rounds = 0;
                 while (true) {
                     for (int r = 0; r < 4; ++r, ++rounds)
-                        k_pair_round<R><<<nblk(npairs, 256), 256, 0, stream>>>(pc);
+                        k_pair_round<R><<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>());
                     pull_scalars();
                     if (hs->undecided <= 0) break;
                     if (rounds > 4 * (npairs + 8))

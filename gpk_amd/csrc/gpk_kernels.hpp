@@ -15,6 +15,13 @@ namespace gpkd {
 
 enum : int { UNDEC = 0, ACCEPT = 1, REJECT = 2, QUEUED = 3 };
 
+// LDS written by some lanes of a wavefront and then read by others
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ int ld_acq(const int* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -1419,86 +1426,156 @@ __device__ __forceinline__ void pair_view(const PairCtx& c, int cell, int q, int
     gv = g[cell];
 }
 
-__device__ __forceinline__ int32_t pair_label(const PairCtx& c, int cell, int q) {
-    const int p = c.pid[cell];
-    if (p >= 0 && p < q && ld_acq(c.pstate + p) == ACCEPT)
-        return c.Pa[p] == cell ? c.Phi[p] : c.Plo[p];
-    return c.phi[cell];
+// Readiness set of each pair: the earlier pairs that own a cell inside the
+// 5^d boxes around a and b and share one of its two labels.  The set depends
+// only on the pair table, so it is listed once per sweep, one wavefront per
+// pair (lane l looks at box cells l, l + 64, ...), in box order.  Entry
+// q * PAIR_DEPS holds the count, or -1 when the list does not fit (then the
+// round kernel scans the boxes itself).
+constexpr int PAIR_DEPS = 16;
+
+__global__ void __launch_bounds__(256) k_pair_deps(PairCtx c, int* __restrict__ deps) {
+    const int q = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= c.np) return;
+    const Dims& d = c.d;
+    const bool d3 = c.dim3 != 0;
+    const int a = c.Pa[q], b = c.Pb[q];
+    const int32_t lo = c.Plo[q], hi = c.Phi[q];
+    const int nbox = d3 ? 125 : 25;
+    int za, ya, xa, zb, yb, xb;
+    decode(d, a, za, ya, xa);
+    decode(d, b, zb, yb, xb);
+    int n = 0;
+    for (int k0 = 0; k0 < 2 * nbox; k0 += 64) {
+        const int k = k0 + lane;
+        bool hit = false;
+        int p = -1;
+        if (k < 2 * nbox) {
+            const int side = k >= nbox ? 1 : 0;
+            const int r = k - side * nbox;
+            const int lz = d3 ? r / 25 : 2, ly = (r / 5) % 5, lx = r % 5;
+            const int z0 = side ? zb : za, y0 = side ? yb : ya, x0 = side ? xb : xa;
+            const int cell = lin(d, wrapi(z0 + lz - 2, d.Nz), wrapi(y0 + ly - 2, d.Ny), wrapi(x0 + lx - 2, d.Nx));
+            p = c.pid[cell];
+            if (p >= 0 && p < q) {
+                const int32_t pl = c.Plo[p], ph = c.Phi[p];
+                hit = pl == lo || pl == hi || ph == lo || ph == hi;
+            }
+        }
+        const unsigned long long m = __ballot(hit);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (hit && n + before < PAIR_DEPS - 1) deps[q * PAIR_DEPS + 1 + n + before] = p;
+        n += __popcll(m);
+    }
+    if (lane == 0) deps[q * PAIR_DEPS] = n < PAIR_DEPS ? n : -1;
 }
 
-// bounded certificate for relabelling `cell` old->nl, with one override
-// (cell ov carries label ovl), src/gpk_connectivity.cpp:10-195 (local_only)
-__device__ bool bounded_move(const PairCtx& c, int q, int cell, int32_t ol, int32_t nl,
-                             int old_count, int ov, int32_t ovl) {
-    return bounded_cert(c.d, c.dim3 != 0, cell, ol, nl, old_count, [&](int cc) -> int32_t {
-        return cc == ov ? ovl : (cc == cell ? ol : pair_label(c, cc, q));
-    });
-}
-
+// One wavefront per pair q (src/gpk_transitions.cpp:129-170 on the versioned
+// view).  Lanes check the readiness list, then gather the versioned label and
+// orientation of every cell of the 5^d boxes around a and b in parallel; the
+// certificates' box masks come from ballots, and lane 0 takes the decision.
 template <class R>
-__global__ void __launch_bounds__(256) k_pair_round(PairCtx c) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __restrict__ deps) {
+    constexpr int WPB = 4;                       // wavefronts (pairs) per block
+    __shared__ int32_t blab[WPB][2][128];        // versioned labels, box A / box B
+    __shared__ R bg[WPB][2][128];                // versioned orientations
+    __shared__ int bcell[WPB][2][128];           // grid index of each box position
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int q = blockIdx.x * WPB + w;
     if (q >= c.np) return;
     if (ld_acq(c.pstate + q) != UNDEC) return;
     const Dims& d = c.d;
     const bool d3 = c.dim3 != 0;
     const int a = c.Pa[q], b = c.Pb[q];
     const int32_t lo = c.Plo[q], hi = c.Phi[q];
+    const int nbox = d3 ? 125 : 25;
+    int za, ya, xa, zb, yb, xb;
+    decode(d, a, za, ya, xa);
+    decode(d, b, zb, yb, xb);
     // readiness: earlier pairs sharing a label inside the 5^d boxes of a and b
-    for (int side = 0; side < 2; ++side) {
-        int z0, y0, x0;
-        decode(d, side == 0 ? a : b, z0, y0, x0);
-        const int nz = d3 ? 5 : 1;
-        for (int lz = 0; lz < nz; ++lz) {
-            const int z = wrapi(z0 + (d3 ? lz - 2 : 0), d.Nz);
-            for (int ly = 0; ly < 5; ++ly) {
-                const int y = wrapi(y0 + ly - 2, d.Ny);
-                for (int lx = 0; lx < 5; ++lx) {
-                    const int x = wrapi(x0 + lx - 2, d.Nx);
-                    const int p = c.pid[lin(d, z, y, x)];
-                    if (p >= 0 && p < q) {
-                        const int32_t pl = c.Plo[p], ph = c.Phi[p];
-                        if ((pl == lo || pl == hi || ph == lo || ph == hi) &&
-                            ld_acq(c.pstate + p) == UNDEC) return;
-                    }
-                }
+    const int nd = deps[q * PAIR_DEPS];
+    bool wait = false;
+    if (nd >= 0) {
+        if (lane < nd) wait = ld_acq(c.pstate + deps[q * PAIR_DEPS + 1 + lane]) == UNDEC;
+    } else {
+        for (int k = lane; k < 2 * nbox; k += 64) {
+            const int side = k >= nbox ? 1 : 0;
+            const int r = k - side * nbox;
+            const int lz = d3 ? r / 25 : 2, ly = (r / 5) % 5, lx = r % 5;
+            const int z0 = side ? zb : za, y0 = side ? yb : ya, x0 = side ? xb : xa;
+            const int cell = lin(d, wrapi(z0 + lz - 2, d.Nz), wrapi(y0 + ly - 2, d.Ny), wrapi(x0 + lx - 2, d.Nx));
+            const int p = c.pid[cell];
+            if (p >= 0 && p < q) {
+                const int32_t pl = c.Plo[p], ph = c.Phi[p];
+                if ((pl == lo || pl == hi || ph == lo || ph == hi) && ld_acq(c.pstate + p) == UNDEC)
+                    wait = true;
             }
         }
     }
+    if (__ballot(wait) != 0ull) return;
+    // versioned view of both boxes
+    for (int k = lane; k < 2 * 128; k += 64) {
+        const int side = k >> 7, r = k & 127;
+        if (r < nbox) {
+            const int lz = d3 ? r / 25 : 2, ly = (r / 5) % 5, lx = r % 5;
+            const int z0 = side ? zb : za, y0 = side ? yb : ya, x0 = side ? xb : xa;
+            const int cell = lin(d, wrapi(z0 + lz - 2, d.Nz), wrapi(y0 + ly - 2, d.Ny), wrapi(x0 + lx - 2, d.Nx));
+            int32_t l;
+            R gv;
+            pair_view<R>(c, cell, q, l, gv);
+            blab[w][side][r] = l;
+            bg[w][side][r] = gv;
+            bcell[w][side][r] = cell;
+        }
+    }
+    wave_sync();
+    // box masks of the two certificates: relabel a lo -> hi with a's own
+    // label read as lo; relabel b hi -> lo with a (already moved) and b read
+    // as hi (src/gpk_transitions.cpp:151-158)
+    Bits128 boA{0, 0}, bnA{0, 0}, boB{0, 0}, bnB{0, 0};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = lane + 64 * h;
+        bool oA = false, nA = false, oB = false, nB = false;
+        if (r < nbox) {
+            const int ca = bcell[w][0][r], cb = bcell[w][1][r];
+            const int32_t la = ca == a ? lo : blab[w][0][r];
+            const int32_t lb = cb == a ? hi : (cb == b ? hi : blab[w][1][r]);
+            oA = la == lo; nA = la == hi;
+            oB = lb == hi; nB = lb == lo;
+        }
+        const unsigned long long mo = __ballot(oA), mn = __ballot(nA);
+        const unsigned long long po = __ballot(oB), pn = __ballot(nB);
+        if (h == 0) { boA.lo = mo; bnA.lo = mn; boB.lo = po; bnB.lo = pn; }
+        else { boA.hi = mo; bnA.hi = mn; boB.hi = po; bnB.hi = pn; }
+    }
+    if (lane != 0) return;
     R* Pg = static_cast<R*>(c.Pg);
     const int nface = d3 ? 6 : 4;
-    int32_t la, lb;
-    R gva, gvb;
-    pair_view<R>(c, a, q, la, gva);
-    pair_view<R>(c, b, q, lb, gvb);
+    const int lc = (d3 ? 2 : 0) * 25 + 12;
+    const int fo[6] = {lc - 1, lc + 1, lc - 5, lc + 5, lc - 25, lc + 25};
+    const int32_t la = blab[w][0][lc], lb = blab[w][1][lc];
     int verdict = REJECT;
+    R ga = bg[w][0][lc], gb = bg[w][1][lc];
     if (la == lo && lb == hi) {
-        int fa[6], fb[6];
-        faces(d, a, fa);
-        faces(d, b, fb);
         // target orientations (first face neighbour carrying the target label)
-        R ga = gva, gb = gvb;
-        for (int k = 0; k < nface; ++k) {
-            int32_t l; R gv; pair_view<R>(c, fa[k], q, l, gv);
-            if (l == hi) { ga = gv; break; }
-        }
-        for (int k = 0; k < nface; ++k) {
-            int32_t l; R gv; pair_view<R>(c, fb[k], q, l, gv);
-            if (l == lo) { gb = gv; break; }
-        }
-        auto mism = [&](const int* f, int32_t prop, int pc, int32_t pp) {
+        for (int k = 0; k < nface; ++k)
+            if (blab[w][0][fo[k]] == hi) { ga = bg[w][0][fo[k]]; break; }
+        for (int k = 0; k < nface; ++k)
+            if (blab[w][1][fo[k]] == lo) { gb = bg[w][1][fo[k]]; break; }
+        auto mism = [&](int side, int32_t prop, int pc, int32_t pp) {
             int m = 0;
             for (int k = 0; k < nface; ++k) {
-                const int32_t l = f[k] == pc ? pp : pair_label(c, f[k], q);
+                const int32_t l = bcell[w][side][fo[k]] == pc ? pp : blab[w][side][fo[k]];
                 m += l != prop;
             }
             return m;
         };
-        const int before = mism(fa, lo, b, hi) + mism(fb, hi, a, lo);
-        const int after = mism(fa, hi, b, lo) + mism(fb, lo, a, hi);
-        if (after < before &&
-            bounded_move(c, q, a, lo, hi, c.count[lo], -1, 0) &&
-            bounded_move(c, q, b, hi, lo, c.count[hi] + 1, a, hi)) {
+        const int before = mism(0, lo, b, hi) + mism(1, hi, a, lo);
+        const int after = mism(0, hi, b, lo) + mism(1, lo, a, hi);
+        if (after < before && bounded_cert_masks(d3, boA, bnA, c.count[lo]) &&
+            bounded_cert_masks(d3, boB, bnB, c.count[hi] + 1)) {
             verdict = ACCEPT;
             st_agent(Pg + 2 * q, ga);
             st_agent(Pg + 2 * q + 1, gb);
