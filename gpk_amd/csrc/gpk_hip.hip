@@ -269,7 +269,8 @@ struct Engine final : EngineBase {
     DevBuf Pk, Ps;
     DevBuf gain_s, pcell, kgain, kgain2, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
         pflag2, pidx;
-    DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps;
+    DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt;
+    bool mask_fresh = false;    // mask matches the current labels
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
@@ -281,7 +282,7 @@ struct Engine final : EngineBase {
     // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
     // flags / list of the interface labels, hipcub scratch for the label list
     bool broad = false;
-    DevBuf kw, lflag, llist, lcub;
+    DevBuf kw, lflag, llist;
     std::vector<int32_t> hlabels;
     DevBuf scal, dsum, tagctr, minmax, noise_amp;
     DevBuf cub_tmp;
@@ -337,12 +338,12 @@ struct Engine final : EngineBase {
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
                                &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain, &kgain2,
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
-                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &count, &unres, &ulist,
+                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &cmp_cnt, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
                                &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
                                &rs_gain, &rs_best, &rs_valid, &rs_dirty,
                                &eps_b, &eps_c,
-                               &llist, &lcub};
+                               &llist};
         for (auto* b : all) t += b->bytes;
         return t;
     }
@@ -423,10 +424,7 @@ struct Engine final : EngineBase {
         HIPCHK(hipMemcpy(projw.p, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(remapw.p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
         // hipcub scratch: sized for the largest call
-        size_t b1 = 0, b2 = 0, b3 = 0, b4 = 0;
-        hipcub::CountingInputIterator<int> it(0);
-        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, b1, it, mask.as<uint8_t>(), sel.as<int>(),
-                                             &scal.as<Scalars>()->nsel, N, stream));
+        size_t b2 = 0, b3 = 0, b4 = 0;
         HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, Pk.as<unsigned long long>(),
                                                  Ps.as<unsigned long long>(), 2 * N, 0, 64, stream));
         HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, kgain.as<unsigned long long>(),
@@ -434,7 +432,7 @@ struct Engine final : EngineBase {
                                                   order.as<int>(), N, 0, 64, stream));
         HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b4, pflag2.as<int>(), pidx.as<int>(), N,
                                                 stream));
-        cub_bytes = std::max({b1, b2, b3, b4}) + 256;
+        cub_bytes = std::max({b2, b3, b4}) + 256;
         cub_tmp.alloc(cub_bytes);
         // initial values (src/gpk_grid.cpp:14-21)
         fill<int32_t>(phi, N, 0);
@@ -525,6 +523,7 @@ struct Engine final : EngineBase {
         if (!src || bytes != size_t(N) * elem(kind)) throw GpkError(GPK_E_INVALID, "upload size mismatch");
         HIPCHK(hipMemcpyAsync(b->p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                               stream));
+        mask_fresh = false;
         if (f == GPK_FIELD_PHI) {
             check_labels();
             rs_reset = true;   // labels written from outside: no cached remap score holds
@@ -646,13 +645,7 @@ struct Engine final : EngineBase {
         k_fill_u8<<<gsblk(L, 256), 256, 0, stream>>>(L, lflag.as<uint8_t>(), 0);
         k_broad_flag_labels<<<nblk(nb, 256), 256, 0, stream>>>(nb, bandA.as<int>(), phi.as<int32_t>(),
                                                                lflag.as<uint8_t>());
-        hipcub::CountingInputIterator<int> it(0);
-        size_t tb = 0;
-        HIPCHK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, lflag.as<uint8_t>(), llist.as<int>(),
-                                             &ds()->nsel, L, stream));
-        if (lcub.bytes < tb) lcub.alloc(tb);
-        HIPCHK(hipcub::DeviceSelect::Flagged(lcub.p, tb, it, lflag.as<uint8_t>(), llist.as<int>(),
-                                             &ds()->nsel, L, stream));
+        compact_async(lflag.as<uint8_t>(), L, llist.as<int>());
         pull_scalars();
         const int ng = hs->nsel;
         hlabels.resize(ng);
@@ -702,13 +695,27 @@ struct Engine final : EngineBase {
         f(ntx, nty, ntz, 8 * ((ntiles + 7) / 8));
     }
 
+    // ---- ordered compaction of a byte-flag array (count, scan, scatter);
+    // leaves the count in the device scalars (nsel) without reading it back
+    void compact_async(const uint8_t* flags, long long n, int* out) {
+        const int nb = int((n + CMP_CHUNK - 1) / CMP_CHUNK);
+        if (cmp_cnt.bytes < size_t(nb) * 4) cmp_cnt.alloc(size_t(nb) * 4);
+        k_cmp_count<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>());
+        k_cmp_scan<<<1, 1024, 0, stream>>>(nb, cmp_cnt.as<int>(), &ds()->nsel);
+        k_cmp_scatter<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>(), out);
+    }
+
+    // ---- interface mask (src/gpk_geometry.cpp:42-82); mask_fresh records
+    // that mask matches the current labels, so a compaction can skip it
+    void update_mask() {
+        k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
+        mask_fresh = true;
+    }
+
     // ---- band compaction: mask + sorted interface list (+ slot map) ----
     int band_compact(DevBuf& list, bool with_slot) {
-        k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
-        hipcub::CountingInputIterator<int> it(0);
-        size_t tb = cub_bytes;
-        HIPCHK(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, mask.as<uint8_t>(), list.as<int>(),
-                                             &ds()->nsel, N, stream));
+        if (!mask_fresh) update_mask();
+        compact_async(mask.as<uint8_t>(), N, list.as<int>());
         pull_scalars();
         const int nb = hs->nsel;
         if (with_slot && nb > 0)
@@ -718,9 +725,7 @@ struct Engine final : EngineBase {
 
     int select_flags(const uint8_t* flags, int n, int* out) {
         if (n <= 0) return 0;
-        hipcub::CountingInputIterator<int> it(0);
-        size_t tb = cub_bytes;
-        HIPCHK(hipcub::DeviceSelect::Flagged(cub_tmp.p, tb, it, flags, out, &ds()->nsel, n, stream));
+        compact_async(flags, n, out);
         pull_scalars();
         return hs->nsel;
     }
@@ -989,6 +994,7 @@ struct Engine final : EngineBase {
                     throw GpkError(GPK_E_HIP, "flip transaction did not converge");
             }
         }
+        mask_fresh = false;
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
             phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
@@ -1087,6 +1093,7 @@ struct Engine final : EngineBase {
                     if (rounds > 4 * (npairs + 8))
                         throw GpkError(GPK_E_HIP, "remap transaction did not converge");
                 }
+                mask_fresh = false;
                 k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
                     npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
                     Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds(),
@@ -1094,7 +1101,7 @@ struct Engine final : EngineBase {
                     rs_dirty.as<uint8_t>());
             }
         }
-        band_compact(bandB, false);
+        if (!mask_fresh) update_mask();
         pull_scalars();
         return hs->exchanges;
     }
@@ -1113,13 +1120,14 @@ struct Engine final : EngineBase {
         k_unres_flags<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), unres.as<uint8_t>(),
                                                          uflag.as<uint8_t>());
         const int n = select_flags(uflag.as<uint8_t>(), N, ulist.as<int>());
+        mask_fresh = false;
         if (n > 0)
             k_erode<R><<<1, 1024, 0, stream>>>(dims, dim3, n, ulist.as<int>(), unres.as<uint8_t>(),
                                                phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(),
                                                utgt.as<int>(), utgg.as<R>(), ds(), rho.as<R>(),
                                                R(model.rho_rex), sweep_rho(), remap_r,
                                                rs_dirty.as<uint8_t>());
-        band_compact(bandB, false);
+        update_mask();
     }
 
     // ---- partitioned solute transport (src/gpk_transport.cpp:48-135) ----
@@ -1264,6 +1272,7 @@ struct Engine final : EngineBase {
         if (n_seeds <= 0 || n_seeds > N)
             throw GpkError(GPK_E_INVALID, "Voronoi seed count must lie between one and the cell count");
         rs_reset = true;
+        mask_fresh = false;
         Sites S = draw_sites(Nz, Ny, Nx, n_seeds, seed);
         std::vector<int32_t> h_phi(N);
         // nearest site on the device

@@ -43,6 +43,88 @@ __global__ void k_mask(Dims d, const int32_t* __restrict__ phi, uint8_t* __restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Ordered stream compaction: out = indices i (ascending) with flags[i] != 0.
+// Block b owns items [4096 b, 4096 b + 4096), thread t the 16 consecutive
+// items at 16 t; count per block, one-block exclusive scan of the block
+// counts, then each thread writes its set items at its block offset plus the
+// prefix of its block's thread counts.
+// ---------------------------------------------------------------------------
+constexpr int CMP_T = 256, CMP_V = 16, CMP_CHUNK = CMP_T * CMP_V;
+
+__device__ __forceinline__ uint32_t flag_bits16(const uint8_t* __restrict__ f, long long base, long long n) {
+    uint32_t bits = 0;
+    if (base + CMP_V <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(f + base);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < CMP_V; ++j) bits |= ((w[j >> 2] >> (8 * (j & 3))) & 0xFFu) ? (1u << j) : 0u;
+    } else {
+        for (int j = 0; j < CMP_V; ++j)
+            if (base + j < n && f[base + j]) bits |= 1u << j;
+    }
+    return bits;
+}
+
+// inclusive prefix sum over the block's threads in thread order; *total gets
+// the block sum (LDS: one int per wavefront + the total)
+template <int NT>
+__device__ __forceinline__ int block_incl_scan(int v, int* wsum, int* total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int u = __shfl_up(v, off);
+        if (lane >= off) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int k = 0; k < NT / 64; ++k) { const int t = wsum[k]; wsum[k] = acc; acc += t; }
+        *total = acc;
+    }
+    __syncthreads();
+    return v + wsum[w];
+}
+
+__global__ void __launch_bounds__(CMP_T) k_cmp_count(long long n, const uint8_t* __restrict__ f,
+                                                     int* __restrict__ bcnt) {
+    __shared__ int wsum[CMP_T / 64], tot;
+    const long long base = (long long)blockIdx.x * CMP_CHUNK + (long long)threadIdx.x * CMP_V;
+    const int c = __popc(flag_bits16(f, base, n));
+    (void)block_incl_scan<CMP_T>(c, wsum, &tot);
+    if (threadIdx.x == 0) bcnt[blockIdx.x] = tot;
+}
+
+// one block: bcnt (counts) -> exclusive offsets, *total = sum
+__global__ void __launch_bounds__(1024) k_cmp_scan(int nb, int* __restrict__ bcnt, int* __restrict__ total) {
+    __shared__ int wsum[16], tot;
+    int running = 0;
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < nb ? bcnt[b] : 0;
+        const int incl = block_incl_scan<1024>(v, wsum, &tot);
+        if (b < nb) bcnt[b] = running + incl - v;
+        running += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = running;
+}
+
+__global__ void __launch_bounds__(CMP_T) k_cmp_scatter(long long n, const uint8_t* __restrict__ f,
+                                                       const int* __restrict__ boff, int* __restrict__ out) {
+    __shared__ int wsum[CMP_T / 64], tot;
+    const long long base = (long long)blockIdx.x * CMP_CHUNK + (long long)threadIdx.x * CMP_V;
+    uint32_t bits = flag_bits16(f, base, n);
+    const int c = __popc(bits);
+    int o = boff[blockIdx.x] + block_incl_scan<CMP_T>(c, wsum, &tot) - c;
+    while (bits) {
+        const int j = __ffs(bits) - 1;
+        out[o++] = int(base + j);
+        bits &= bits - 1u;
+    }
+}
+
 __global__ void k_slot(int n, const int* __restrict__ band, int* __restrict__ slot) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a < n) slot[band[a]] = a;
