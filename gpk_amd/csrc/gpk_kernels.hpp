@@ -280,6 +280,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     __shared__ short hidx[GEO_HCAP];
     __shared__ int resl[GEO_HCAP];
     __shared__ int bcount[S::MAXB + 1], boff[S::MAXB + 1];
+    __shared__ int zoff[S::SZ], yoff[S::SY];    // wrapped plane / row offsets of the staged rows
     __shared__ int nlist, nres;
 
     const int ntiles = ntx * nty * ntz;
@@ -296,6 +297,8 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     for (int e = threadIdx.x; e < S::W; e += blockDim.x) {
         wy[0][e] = w0[e]; wy[1][e] = w1[e]; wy[2][e] = w2[e];
     }
+    for (int e = threadIdx.x; e < S::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - S::HZ, d.Nz) * d.Ny * d.Nx;
+    for (int e = threadIdx.x; e < S::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RG, d.Ny) * d.Nx;
     __syncthreads();
     // interface cells of the tile and their distinct resident labels
     for (int e = threadIdx.x; e < NT; e += blockDim.x) {
@@ -353,6 +356,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     }
     const int nr = nres;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    const int xw = lane < S::SX ? wrapi(x0t + lane - RG, d.Nx) : 0;
     constexpr int INFL = 4;               // staged rows in flight per wavefront
     for (int b0 = 0; b0 < nr; b0 += S::MAXRES) {
         const int nbat = nr - b0 < S::MAXRES ? nr - b0 : S::MAXRES;
@@ -365,11 +369,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
             for (int k = 0; k < INFL; ++k) {
                 const int r = r0 + k * nwave;
                 v[k] = -1;
-                if (r < S::NROWS && lane < S::SX) {
-                    const int lz = r / S::SY, ly = r % S::SY;
-                    v[k] = phi[lin(d, wrapi(z0t + lz - S::HZ, d.Nz), wrapi(y0t + ly - RG, d.Ny),
-                                   wrapi(x0t + lane - RG, d.Nx))];
-                }
+                if (r < S::NROWS && lane < S::SX) v[k] = phi[zoff[r / S::SY] + yoff[r % S::SY] + xw];
             }
 #pragma unroll
             for (int k = 0; k < INFL; ++k) {
@@ -498,6 +498,7 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     __shared__ uint32_t key[H::NS];
     __shared__ R vv[H::NS];
     __shared__ short list[H::NT];
+    __shared__ int zoff[H::SZ], yoff[H::SY], xoff[H::SX];   // wrapped offsets of the staged box
     __shared__ int nlist;
     const int ntiles = ntx * nty * ntz;
     const int tile = xcd_tile(ntiles);
@@ -505,6 +506,10 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
     if (threadIdx.x == 0) nlist = 0;
+    for (int e = threadIdx.x; e < H::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - H::HZ, d.Nz) * d.Ny * d.Nx;
+    for (int e = threadIdx.x; e < H::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RP, d.Ny) * d.Nx;
+    for (int e = threadIdx.x; e < H::SX; e += blockDim.x) xoff[e] = wrapi(x0t + e - RP, d.Nx);
+    __syncthreads();
     // key = ((low << 16 | high) << 1 | [resident is the high label]) for
     // interface cells whose two labels differ, all ones otherwise (labels
     // below 2^15, so the packed pair fits 31 bits and never equals all ones).
@@ -520,8 +525,7 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
             const int e = e0 + k * blockDim.x;
             const int ee = e < H::NS ? e : 0;
             const int lx = ee % H::SX, ly = (ee / H::SX) % H::SY, lz = ee / (H::SX * H::SY);
-            j[k] = lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RP, d.Ny),
-                       wrapi(x0t + lx - RP, d.Nx));
+            j[k] = zoff[lz] + yoff[ly] + xoff[lx];
         }
 #pragma unroll
         for (int k = 0; k < INFL; ++k) {
@@ -679,6 +683,7 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
     using H = HaloGeom<DIM, RS>;
     __shared__ uint16_t lab[H::NS];
     __shared__ short list[H::NT];
+    __shared__ int zoff[H::SZ], yoff[H::SY], xoff[H::SX];   // wrapped offsets of the staged box
     __shared__ int nlist;
     const int ntiles = ntx * nty * ntz;
     const int tile = xcd_tile(ntiles);
@@ -709,10 +714,13 @@ k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict
     __syncthreads();
     const int n = nlist;
     if (n == 0) return;
+    for (int e = threadIdx.x; e < H::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - H::HZ, d.Nz) * d.Ny * d.Nx;
+    for (int e = threadIdx.x; e < H::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RS, d.Ny) * d.Nx;
+    for (int e = threadIdx.x; e < H::SX; e += blockDim.x) xoff[e] = wrapi(x0t + e - RS, d.Nx);
+    __syncthreads();
     for (int e = threadIdx.x; e < H::NS; e += blockDim.x) {
         const int lx = e % H::SX, ly = (e / H::SX) % H::SY, lz = e / (H::SX * H::SY);
-        lab[e] = uint16_t(phi[lin(d, wrapi(z0t + lz - H::HZ, d.Nz), wrapi(y0t + ly - RS, d.Ny),
-                                  wrapi(x0t + lx - RS, d.Nx))]);
+        lab[e] = uint16_t(phi[zoff[lz] + yoff[ly] + xoff[lx]]);
     }
     __syncthreads();
     const int nface = DIM == 3 ? 6 : 4;
