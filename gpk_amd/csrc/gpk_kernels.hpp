@@ -195,7 +195,8 @@ struct TiledLds {
 template <class R, int DIM, int RG, int LAW, class RowMask>
 __device__ __forceinline__ void tiled_cell(
     const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0,
-    int32_t p, const R* lut, const R (*wy)[2 * RG + 1], RowMask rowmask,
+    int32_t p, const R* lut, const R* __restrict__ w0, const R* __restrict__ w1,
+    const R* __restrict__ w2, RowMask rowmask,
     const int32_t* __restrict__ phi, const int* __restrict__ slot, const R* __restrict__ g,
     const R* __restrict__ c, const R* __restrict__ T, const R* __restrict__ rho,
     const R* __restrict__ E_el, R* __restrict__ onx, R* __restrict__ ony,
@@ -222,13 +223,13 @@ __device__ __forceinline__ void tiled_cell(
                 else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
                 base += (1 << len) * 3;
             }
-            const R gy0 = wy[0][jy], gy1 = wy[1][jy], gy2 = wy[2][jy];
+            const R gy0 = w0[jy], gy1 = w1[jy], gy2 = w2[jy];
             S00 = fmar(gy0, R0, S00); S01 = fmar(gy0, R1, S01);
             S02 = fmar(gy0, R2, S02); S10 = fmar(gy1, R0, S10);
             S11 = fmar(gy1, R1, S11); S20 = fmar(gy2, R0, S20);
         }
         if (DIM == 3) {
-            const R gz0 = wy[0][jz], gz1 = wy[1][jz], gz2 = wy[2][jz];
+            const R gz0 = w0[jz], gz1 = w1[jz], gz2 = w2[jz];
             Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
             Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
             Ixy = fmar(gz0, S11, Ixy);
@@ -274,7 +275,6 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     short* const list = reinterpret_cast<short*>(rm);
     unsigned short* const cres = reinterpret_cast<unsigned short*>(rm) + NT;
     __shared__ R lut[S::NLUT];
-    __shared__ R wy[3][S::W];
     __shared__ uint16_t blist[NT];        // (slot in the batch << 12) | tile cell, grouped by batch
     __shared__ int hkey[GEO_HCAP];
     __shared__ short hidx[GEO_HCAP];
@@ -294,9 +294,6 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     for (int e = threadIdx.x; e < GEO_HCAP; e += blockDim.x) hkey[e] = -1;
     for (int e = threadIdx.x; e <= S::MAXB; e += blockDim.x) bcount[e] = 0;
     for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e] = lut_g[e];
-    for (int e = threadIdx.x; e < S::W; e += blockDim.x) {
-        wy[0][e] = w0[e]; wy[1][e] = w1[e]; wy[2][e] = w2[e];
-    }
     for (int e = threadIdx.x; e < S::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - S::HZ, d.Nz) * d.Ny * d.Nx;
     for (int e = threadIdx.x; e < S::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RG, d.Ny) * d.Nx;
     __syncthreads();
@@ -390,7 +387,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
             const int t = int(pk >> 12);
             const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
             const unsigned long long* rows = rm + (lz * S::SY + ly) * S::MAXRES + t;
-            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[b0 + t], lut, wy,
+            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[b0 + t], lut, w0, w1, w2,
                                    [&](int jz, int jy) {
                                        return rows[(jz * S::SY + jy) * S::MAXRES] >> lx;
                                    },
@@ -404,7 +401,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
         const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
         const int32_t p = phi[lin(d, z0, y0, x0)];
-        tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0, y0, z0, p, lut, wy,
+        tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0, y0, z0, p, lut, w0, w1, w2,
                                [&](int jz, int jy) {
                                    const int z = DIM == 3 ? wrapi(z0 + jz - RG, d.Nz) : z0;
                                    const int y = wrapi(y0 + jy - RG, d.Ny);
