@@ -187,6 +187,10 @@ struct TiledLds {
         return n;
     }
     static constexpr int NLUT = lut_entries();
+    // LDS stride of one table entry (3 sums): FP32 entries are padded to 16
+    // bytes so that one 128-bit LDS read returns all three
+    static constexpr int LS = sizeof(R) == 4 ? 4 : 3;
+    static constexpr int NLUT_LDS = NLUT / 3 * LS;
     static_assert(SX <= 64, "a staged row must fit one wavefront");
 };
 
@@ -218,10 +222,17 @@ __device__ __forceinline__ void tiled_cell(
                 const int start = k * S::C;
                 const int len = (S::W - start) < S::C ? (S::W - start) : S::C;
                 const int ix = int((m >> start) & ((1ull << len) - 1ull));
-                const R* t = lut + base + ix * 3;
-                if (k == 0) { R0 = t[0]; R1 = t[1]; R2 = t[2]; }
-                else { R0 += t[0]; R1 += t[1]; R2 += t[2]; }
-                base += (1 << len) * 3;
+                R t0, t1, t2;
+                if constexpr (S::LS == 4) {
+                    const float4 v = *reinterpret_cast<const float4*>(lut + base + ix * 4);
+                    t0 = v.x; t1 = v.y; t2 = v.z;
+                } else {
+                    const R* t = lut + base + ix * S::LS;
+                    t0 = t[0]; t1 = t[1]; t2 = t[2];
+                }
+                if (k == 0) { R0 = t0; R1 = t1; R2 = t2; }
+                else { R0 += t0; R1 += t1; R2 += t2; }
+                base += (1 << len) * S::LS;
             }
             const R gy0 = w0[jy], gy1 = w1[jy], gy2 = w2[jy];
             S00 = fmar(gy0, R0, S00); S01 = fmar(gy0, R1, S01);
@@ -274,7 +285,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     __shared__ unsigned long long rm[NRM];
     short* const list = reinterpret_cast<short*>(rm);
     unsigned short* const cres = reinterpret_cast<unsigned short*>(rm) + NT;
-    __shared__ R lut[S::NLUT];
+    __shared__ __align__(16) R lut[S::NLUT_LDS];
     __shared__ uint16_t blist[NT];        // (slot in the batch << 12) | tile cell, grouped by batch
     __shared__ int hkey[GEO_HCAP];
     __shared__ short hidx[GEO_HCAP];
@@ -293,7 +304,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     if (threadIdx.x == 0) { nlist = 0; nres = 0; }
     for (int e = threadIdx.x; e < GEO_HCAP; e += blockDim.x) hkey[e] = -1;
     for (int e = threadIdx.x; e <= S::MAXB; e += blockDim.x) bcount[e] = 0;
-    for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e] = lut_g[e];
+    for (int e = threadIdx.x; e < S::NLUT; e += blockDim.x) lut[e / 3 * S::LS + e % 3] = lut_g[e];
     for (int e = threadIdx.x; e < S::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - S::HZ, d.Nz) * d.Ny * d.Nx;
     for (int e = threadIdx.x; e < S::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RG, d.Ny) * d.Nx;
     __syncthreads();
