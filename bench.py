@@ -78,6 +78,7 @@ def run_gpu(prec, size, grains, sigma, warmup, steps):
         "accepted": int(st.accepted),
         "deferred": int(st.deferred),
         "exchanges": int(st.exchanges),
+        "guard_band_decisions": int(st.guarded),
         "geometry_ms_avg": geo_ms,
         "geometry_launches": launches,
         "geometry_flops_per_launch": flops,
@@ -237,6 +238,7 @@ def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20):
         "ms_per_step": 1e3 * wall / steps, "instance_steps_per_s": n * steps / wall,
         "cell_updates_per_s": st["band_cells"] * steps / (steps + 1) / wall,
         "accepted": int(st["accepted"]), "exchanges": int(st["exchanges"]),
+        "guard_band_decisions": int(st["guarded"]),
         "instance_step_ms_p50": float(np.percentile(per, 50)),
         "instance_step_ms_p99": float(np.percentile(per, 99)),
         "autotune": {"workgroup_size": wg, "workgroups": nwg, "step_ms": tune_ms, "seconds": t_tune},

@@ -78,14 +78,14 @@ class BatchStats(C.Structure):
     """gpk_batch_stats (include/gpk_c.h)."""
     _fields_ = [("band_cells", C.c_int64), ("accepted", C.c_int64), ("exchanges", C.c_int64),
                 ("proposals", C.c_int64), ("instances", C.c_int32), ("workgroup_size", C.c_int32),
-                ("workgroups", C.c_int32)]
+                ("workgroups", C.c_int32), ("guarded", C.c_int64)]
 
 
 class StepStats(C.Structure):
     _fields_ = [
         ("band_cells", C.c_int64), ("candidates", C.c_int64), ("accepted", C.c_int64),
         ("deferred", C.c_int64), ("exchanges", C.c_int64), ("eroded", C.c_int64),
-        ("dt", C.c_double), ("time", C.c_double), ("step_id", C.c_int64),
+        ("dt", C.c_double), ("time", C.c_double), ("step_id", C.c_int64), ("guarded", C.c_int64),
     ]
 
 

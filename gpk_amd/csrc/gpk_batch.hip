@@ -46,7 +46,7 @@ struct Inst {
 
 struct InstStat {
     double time, dt;
-    long long band_total, accepted_total, exchanges_total, proposals_total;
+    long long band_total, accepted_total, exchanges_total, proposals_total, guarded_total;
     int band, accepted, exchanges, proposals;
     long long t0, t1;     // wall clock at the step's start and end
 };
@@ -236,13 +236,16 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
     dt = A.cap < dt ? A.cap : dt;
     // alpha and flips: without the topology test every candidate is accepted
     // (src/gpk_transitions.cpp:251-313)
+    // (FP32 mode: decisions within the guard band taken again in FP64)
     const R idt = R(double(dt)) / dx;
-    int acc = 0;
+    int acc = 0, gd = 0;
     for (int a = tid; a < nb; a += WG) {
         const int i = band[a];
-        R an = alpha[i] + vn[a] * idt;
-        an = R(1) < an ? R(1) : an;
-        if (an <= Eps<R>::flip && phi_comp[i] != phi[i]) {
+        R an;
+        bool guarded;
+        const bool below = alpha_step<R>(alpha[i], vn[a], idt, double(dt), double(dx), !kDirect, an, guarded);
+        gd += guarded ? 1 : 0;
+        if (below && phi_comp[i] != phi[i]) {
             phi[i] = phi_comp[i];
             g[i] = g_comp[i];
             alpha[i] = R(1);
@@ -252,6 +255,7 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
         }
     }
     const int accepted = block_sum_int<WG>(sh, acc);
+    const int nguard = block_sum_int<WG>(sh, gd);
 
     // balanced Gaussian remap (src/gpk_transitions.cpp:38-178), one sweep in 2-D
     int exchanges = 0, np = 0;
@@ -369,6 +373,7 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
         S.accepted_total += accepted;
         S.exchanges_total += exchanges;
         S.proposals_total += np;
+        S.guarded_total += nguard;
         S.t1 = (long long)wall_clock64();
     }
     __syncthreads();
@@ -635,6 +640,7 @@ struct Batch final : BatchBase {
             s->accepted += x.accepted_total;
             s->exchanges += x.exchanges_total;
             s->proposals += x.proposals_total;
+            s->guarded += x.guarded_total;
         }
         s->instances = n_inst;
         s->workgroup_size = wg;

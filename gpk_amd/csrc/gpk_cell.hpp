@@ -27,11 +27,45 @@ template <> struct Eps<float> {
     static constexpr float vmax0 = 1e-30f, big = 1e30f, c_iface = 0.125f, c_diff = 0.4f;
     static constexpr float cl_floor = 1e-6f;
 };
+// FP64 guard band of the FP32 mode's flip decision.  The FP32 value
+// alpha + v * (dt / dx) differs from the exact value of the same FP32 inputs
+// by at most about 2^-23 (|alpha| + |v dt / dx|) <= 1.4e-7 (|alpha| <= 1 and
+// |v dt / dx| <= 1/8 by the timestep rule), so every cell whose FP32 value
+// lies within flip_guard = 2.5e-7 of the threshold is decided again in FP64;
+// the decision then equals the exact one for the FP32 state.
+constexpr float flip_guard = 2.5e-7f;
+
+// alpha_new = min(alpha + v dt/dx, 1) and whether it is at or below the flip
+// threshold (src/gpk_transitions.cpp:262-291); with `guard`, values inside the
+// band are recomputed and decided in FP64 (FP32 mode only)
+template <class R>
+__device__ __forceinline__ bool alpha_step(R alpha, R v, R idt, double dt, double dx, bool guard,
+                                           R& an, bool& guarded);
+
 template <> struct Eps<double> {
     static constexpr double grad = 1e-8, kappa = 1e-6, flip = 1e-6, gain = 1e-7;
     static constexpr double vmax0 = 1e-30, big = 1e30, c_iface = 0.125, c_diff = 0.4;
     static constexpr double cl_floor = 1e-6;
 };
+
+template <class R>
+__device__ __forceinline__ bool alpha_step(R alpha, R v, R idt, double dt, double dx, bool guard,
+                                           R& an, bool& guarded) {
+    an = alpha + v * idt;
+    an = R(1) < an ? R(1) : an;
+    guarded = false;
+    if (sizeof(R) == 4 && guard) {
+        const float d = float(an) - Eps<float>::flip;
+        if ((d < 0.f ? -d : d) < flip_guard) {
+            double a64 = double(alpha) + double(v) * (dt / dx);
+            a64 = 1.0 < a64 ? 1.0 : a64;
+            an = R(a64);
+            guarded = true;
+            return a64 <= double(Eps<float>::flip);
+        }
+    }
+    return an <= Eps<R>::flip;
+}
 
 __device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }

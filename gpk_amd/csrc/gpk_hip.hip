@@ -731,6 +731,9 @@ struct Engine final : EngineBase {
     }
 
     int sweep_rho() const { return (model.flags & GPK_FLAG_DISLOCATION) ? 1 : 0; }
+    // the FP32 mode decides flips near the threshold again in FP64; strict32
+    // keeps the reference's float decision
+    static constexpr bool guard_band() { return sizeof(R) == 4 && !kDirect; }
 
     // src/gpk_physics.cpp:10-63; depends on c only, so it runs before the
     // fused geometry + velocity kernels that read E_el
@@ -890,6 +893,7 @@ struct Engine final : EngineBase {
         set_int(&ds()->deferred, 0);
         set_int(&ds()->exchanges, 0);
         set_int(&ds()->eroded, 0);
+        set_int(&ds()->guarded, 0);
         int m = 0;
         if (nb > 0) m = flips(nb);
         pull_scalars();
@@ -928,6 +932,7 @@ struct Engine final : EngineBase {
             st->deferred = hs->deferred;
             st->exchanges = exchanges;
             st->eroded = hs->eroded;
+            st->guarded = hs->guarded;
             st->dt = hs->dt;
             st->time = hs->time;
             st->step_id = step_id;
@@ -948,7 +953,7 @@ struct Engine final : EngineBase {
                                                       alpha.as<R>(), T.as<R>(), phi.as<int32_t>(),
                                                       phi_comp.as<int32_t>(), cflag.as<uint8_t>(),
                                                       anew_s.as<R>(), noise ? 1 : 0, amp,
-                                                      uint64_t(step_id));
+                                                      uint64_t(step_id), guard_band() ? 1 : 0, ds());
         const int m = select_flags(cflag.as<uint8_t>(), nb, sel.as<int>());
         if (m == 0) return 0;
         const bool network = (model.flags & GPK_FLAG_GRAIN_NETWORK) != 0;
@@ -1394,7 +1399,7 @@ int gpk_step_many(gpk_handle* h, int nsteps, float dt, float dt_cap, gpk_step_st
             h->eng->step(dt, dt_cap, &st);
             acc.band_cells += st.band_cells; acc.candidates += st.candidates;
             acc.accepted += st.accepted; acc.deferred += st.deferred;
-            acc.exchanges += st.exchanges; acc.eroded += st.eroded;
+            acc.exchanges += st.exchanges; acc.eroded += st.eroded; acc.guarded += st.guarded;
             acc.dt = st.dt; acc.time = st.time; acc.step_id = st.step_id;
         }
         if (stats) *stats = acc;

@@ -810,6 +810,7 @@ struct Scalars {
     int nsel;                       // generic selection count
     int present, unresolved;        // label census
     int flag_err;
+    int guarded;                    // flip decisions taken again in FP64 (FP32 mode)
 };
 
 template <class R>
@@ -849,7 +850,8 @@ __global__ void k_alpha(int nband, const int* __restrict__ band, const Scalars* 
                         float dx_f, const R* __restrict__ vn, R* __restrict__ alpha,
                         const R* __restrict__ T, const int32_t* __restrict__ phi,
                         const int32_t* __restrict__ phi_comp, uint8_t* __restrict__ cflag,
-                        R* __restrict__ anew_s, int noise, float noise_amp, uint64_t step_id) {
+                        R* __restrict__ anew_s, int noise, float noise_amp, uint64_t step_id,
+                        int guard, Scalars* __restrict__ sw) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= nband) return;
     const int i = band[a];
@@ -860,9 +862,11 @@ __global__ void k_alpha(int nband, const int* __restrict__ band, const Scalars* 
         const R Tm = T[i] < R(1) ? R(1) : T[i];
         v += R(noise_amp) * dsqrt(Tm) * R(normal_dev(i, step_id, 0xabcddcbaULL));
     }
-    R an = alpha[i] + v * idt;
-    an = R(1) < an ? R(1) : an;
-    const bool cand = an <= Eps<R>::flip && phi_comp[i] != phi[i];
+    R an;
+    bool guarded;
+    const bool below = alpha_step<R>(alpha[i], v, idt, double(dt), double(R(dx_f)), guard != 0, an, guarded);
+    if (guarded) atomicAdd(&sw->guarded, 1);
+    const bool cand = below && phi_comp[i] != phi[i];
     if (!cand) alpha[i] = an;
     cflag[a] = cand ? 1 : 0;
     anew_s[a] = an;

@@ -150,6 +150,8 @@ typedef struct gpk_step_stats {
     double  dt;             /* timestep used */
     double  time;           /* simulation time after the step */
     int64_t step_id;        /* steps taken by this handle */
+    int64_t guarded;        /* FP32 mode: flip decisions taken again in FP64 because the FP32
+                               alpha_new lay within 2.5e-7 of the threshold (0 otherwise) */
 } gpk_step_stats;
 
 typedef struct gpk_handle gpk_handle;
@@ -294,6 +296,7 @@ typedef struct gpk_batch_stats {
     int64_t proposals;    /* remap proposals */
     int32_t instances;
     int32_t workgroup_size, workgroups;   /* launch shape in use */
+    int64_t guarded;      /* FP32 mode: flip decisions taken again in FP64 (guard band) */
 } gpk_batch_stats;
 
 int gpk_batch_create(int n_inst, const int* Ny, const int* Nx, float dx, float D, float gaussian_sigma,

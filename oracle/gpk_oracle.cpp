@@ -313,6 +313,7 @@ struct Stats {
     int64_t flip_ready0 = 0;        // candidates with no earlier candidate neighbour
     int64_t bfs_visits = 0;         // cells visited by global BFS
     int64_t bgr_proposals = 0, bgr_pairs = 0, bgr_depth = 0;
+    int64_t guarded = 0;            // fp32 mode: decisions taken again in double
 };
 
 // ---------------------------------------------------------------------------
@@ -337,6 +338,7 @@ struct Grid {
     std::vector<uint8_t> mask;
     std::vector<int> active;
     std::vector<R> alpha_new, eps_star, eps_smooth, solid_old;
+    std::vector<uint8_t> below;     // alpha_new at or below the flip threshold
     double time = 0.0;
     uint64_t step_id = 0;
     Stats st;
@@ -1069,21 +1071,37 @@ struct Grid {
         }
         std::copy(alpha.begin(), alpha.end(), alpha_new.begin());
         const int na = static_cast<int>(active.size());
-        #pragma omp parallel for schedule(static)
+        // fp32 mode: an FP32 alpha_new within flip_guard of the threshold is
+        // recomputed and decided in double (the error of the FP32 value is at
+        // most about 2^-23 (|alpha| + |v dt/dx|) <= 1.4e-7)
+        const bool guard = sizeof(R) == 4 && !kDirect;
+        constexpr float flip_guard = 2.5e-7f;
+        below.assign(N, 0);
+        int64_t nguard = 0;
+        #pragma omp parallel for schedule(static) reduction(+ : nguard)
         for (int a = 0; a < na; ++a) {
             const int i = active[a];
             R v = vn[i];
             if (noise)
                 v += amp * std::sqrt(std::max(T[i], R(1))) * R(normal_dev(i, step_id, 0xabcddcbaULL));
             alpha_new[i] = std::min(alpha[i] + v * idt, R(1));
+            bool b = alpha_new[i] <= R(K<R>::flip_eps);
+            if (guard && std::fabs(float(alpha_new[i]) - K<float>::flip_eps) < flip_guard) {
+                const double a64 = std::min(double(alpha[i]) + double(v) * (double(dt) / double(dx)), 1.0);
+                alpha_new[i] = R(a64);
+                b = a64 <= double(K<float>::flip_eps);
+                ++nguard;
+            }
+            below[i] = b ? 1 : 0;
         }
+        st.guarded = nguard;
         auto counts = histogram();
         st.accepted = 0; st.deferred = 0; st.candidates = 0;
         st.flip_global = 0; st.flip_depth = 0; st.flip_ready0 = 0;
         const bool network = law.d.flags & GPK_FLAG_GRAIN_NETWORK;
         std::unordered_map<int, int> lev;
         for (int i : active) {
-            if (!(alpha_new[i] <= R(K<R>::flip_eps) && phi_comp[i] != phi[i])) {
+            if (!(below[i] && phi_comp[i] != phi[i])) {
                 alpha[i] = alpha_new[i];
                 continue;
             }
@@ -1489,6 +1507,7 @@ struct og_stats {
     int64_t band, candidates, accepted, deferred, exchanges, eroded;
     int64_t flip_global, flip_depth, flip_ready0, bfs_visits;
     int64_t bgr_proposals, bgr_pairs, bgr_depth;
+    int64_t guarded;
 };
 
 void* og_create(int mode, int Nz, int Ny, int Nx, float dx, float D, float sigma,
@@ -1525,6 +1544,7 @@ void og_stats_get(void* h, og_stats* o) {
     o->flip_global = s.flip_global; o->flip_depth = s.flip_depth; o->flip_ready0 = s.flip_ready0;
     o->bfs_visits = s.bfs_visits; o->bgr_proposals = s.bgr_proposals; o->bgr_pairs = s.bgr_pairs;
     o->bgr_depth = s.bgr_depth;
+    o->guarded = s.guarded;
 }
 int og_voronoi(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, float* g) {
     return voronoi(Nz, Ny, Nx, n_seeds, seed, phi, g);

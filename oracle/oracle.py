@@ -33,7 +33,7 @@ class OgStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "band", "candidates", "accepted", "deferred", "exchanges", "eroded",
         "flip_global", "flip_depth", "flip_ready0", "bfs_visits",
-        "bgr_proposals", "bgr_pairs", "bgr_depth")]
+        "bgr_proposals", "bgr_pairs", "bgr_depth", "guarded")]
 
 
 _olib = None

@@ -34,7 +34,9 @@ def test_struct_layouts_match_header():
     # sizes of the POD structs crossing the ABI
     assert C.sizeof(_abi.ModelDesc) == 2 * 4 + 21 * 4 + 6 * 4 + 32 * 4
     assert C.sizeof(_abi.GridDesc) == 8 * 4
-    assert C.sizeof(_abi.StepStats) == 6 * 8 + 2 * 8 + 8
+    assert C.sizeof(_abi.StepStats) == 6 * 8 + 2 * 8 + 8 + 8
+    # four counters, three int32 (padded to 8), the guard-band counter
+    assert C.sizeof(_abi.BatchStats) == 4 * 8 + 3 * 4 + 4 + 8
 
 
 def test_model_defaults_agree_with_python_mirror():
