@@ -1273,29 +1273,76 @@ struct Engine final : EngineBase {
         return st == ACCEPT ? 1 : 0;
     }
 
+    // connectivity normalisation on the device (demos/gpk.cpp:122-189); the
+    // work buffers are free outside a step
+    void normalise_device(int n_seeds, const Sites& S) {
+        DevBuf sg, g32;
+        sg.alloc(size_t(n_seeds) * 4);
+        g32.alloc(size_t(N) * 4);
+        HIPCHK(hipMemcpyAsync(sg.p, S.g.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
+        k_site_g<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), sg.as<float>(), g32.as<float>());
+        int* comp = reinterpret_cast<int*>(work3.p);
+        int* csz = reinterpret_cast<int*>(work4.p);
+        k_cc_init<<<gsblk(N, 256), 256, 0, stream>>>(N, comp);
+        int it = 0;
+        while (true) {
+            set_int(&ds()->changed, 0);
+            for (int r = 0; r < 4; ++r, ++it)
+                k_cc_step<<<gsblk(N, 256), 256, 0, stream>>>(dims, dim3, phi.as<int32_t>(), comp, &ds()->changed);
+            pull_scalars();
+            if (hs->changed == 0) break;
+            if (it > N + 8) throw GpkError(GPK_E_STALLED, "Voronoi component labelling did not converge");
+        }
+        fill<int32_t>(work4, N, 0);
+        k_cc_size<<<gsblk(N, 256), 256, 0, stream>>>(N, comp, csz);
+        unsigned long long* best = Pk.as<unsigned long long>();
+        k_fill_r<unsigned long long><<<gsblk(n_seeds + 1, 256), 256, 0, stream>>>(n_seeds + 1, best, 0ull);
+        k_cc_best<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), comp, csz, best);
+        uint8_t* att = cflag.as<uint8_t>();
+        uint8_t* det = uflag.as<uint8_t>();
+        k_cc_attached<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), comp, best, att, det);
+        const int nu = select_flags(det, N, sel.as<int>());
+        int remaining = nu;
+        while (remaining > 0) {
+            set_int(&ds()->nsel, 0);
+            k_attach_scan<<<nblk(nu, 256), 256, 0, stream>>>(dims, dim3, nu, sel.as<int>(), att,
+                                                             phi.as<int32_t>(), g32.as<float>(),
+                                                             queue0.as<int32_t>(), reinterpret_cast<float*>(work1.p),
+                                                             reinterpret_cast<uint8_t*>(qflags.p));
+            k_attach_apply<<<nblk(nu, 256), 256, 0, stream>>>(nu, sel.as<int>(), queue0.as<int32_t>(),
+                                                              reinterpret_cast<float*>(work1.p),
+                                                              reinterpret_cast<uint8_t*>(qflags.p),
+                                                              phi.as<int32_t>(), g32.as<float>(), att,
+                                                              &ds()->nsel);
+            pull_scalars();
+            if (hs->nsel == 0) throw GpkError(GPK_E_STALLED, "Voronoi connectivity normalization stalled");
+            remaining -= hs->nsel;
+        }
+        k_widen_g<R><<<gsblk(N, 256), 256, 0, stream>>>(N, g32.as<float>(), g.as<R>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     void voronoi(int n_seeds, uint32_t seed) override {
         if (n_seeds <= 0 || n_seeds > N)
             throw GpkError(GPK_E_INVALID, "Voronoi seed count must lie between one and the cell count");
         rs_reset = true;
         mask_fresh = false;
         Sites S = draw_sites(Nz, Ny, Nx, n_seeds, seed);
-        std::vector<int32_t> h_phi(N);
         // nearest site on the device
         DevBuf sz, sy, sx;
         sz.alloc(size_t(n_seeds) * 4); sy.alloc(size_t(n_seeds) * 4); sx.alloc(size_t(n_seeds) * 4);
         HIPCHK(hipMemcpyAsync(sz.p, S.z.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(sy.p, S.y.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(sx.p, S.x.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
-        k_voronoi<<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(dims, n_seeds, sz.as<int>(), sy.as<int>(),
-                                                                sx.as<int>(), phi.as<int32_t>());
-        HIPCHK(hipMemcpyAsync(h_phi.data(), phi.p, size_t(N) * 4, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        std::vector<float> h_g(N);
-        const int rc = normalise_components(Nz, Ny, Nx, n_seeds, S, h_phi.data(), h_g.data());
-        if (rc != GPK_OK) throw GpkError(rc, "Voronoi connectivity normalization stalled");
-        std::vector<R> gr(h_g.begin(), h_g.end());
-        HIPCHK(hipMemcpyAsync(phi.p, h_phi.data(), size_t(N) * 4, hipMemcpyHostToDevice, stream));
-        HIPCHK(hipMemcpyAsync(g.p, gr.data(), size_t(N) * sizeof(R), hipMemcpyHostToDevice, stream));
+        const bool narrow = std::max({Nz, Ny, Nx}) <= 65536;
+        if (narrow)
+            k_voronoi<uint32_t><<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(dims, n_seeds, sz.as<int>(), sy.as<int>(),
+                                                                              sx.as<int>(), phi.as<int32_t>());
+        else
+            k_voronoi<unsigned long long><<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(
+                dims, n_seeds, sz.as<int>(), sy.as<int>(), sx.as<int>(), phi.as<int32_t>());
+        normalise_device(n_seeds, S);
         fill<R>(alpha, N, R(1));
         check_labels();
         HIPCHK(hipStreamSynchronize(stream));

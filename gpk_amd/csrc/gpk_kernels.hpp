@@ -809,7 +809,7 @@ struct Scalars {
     int layer_changed;
     int nsel;                       // generic selection count
     int present, unresolved;        // label census
-    int flag_err;
+    int changed;                    // a component-labelling round lowered some id
     int guarded;                    // flip decisions taken again in FP64 (FP32 mode)
 };
 
@@ -2095,6 +2095,10 @@ __global__ void k_count_pos(int n, const int* __restrict__ cnt, int* __restrict_
 // Voronoi nearest site (demos/gpk.cpp:72-89): exact integer periodic squared
 // distance, lowest site index on ties.
 // ---------------------------------------------------------------------------
+// nearest site (periodic squared distance, strict <: the lowest site index
+// wins ties, demos/gpk.cpp:51-120); U is the distance type, 32-bit when every
+// grid side is at most 65536 (3 * 32768^2 < 2^32)
+template <class U>
 __global__ void k_voronoi(Dims d, int ns, const int* __restrict__ sz, const int* __restrict__ sy,
                           const int* __restrict__ sx, int32_t* __restrict__ phi) {
     extern __shared__ int sh[];
@@ -2104,7 +2108,7 @@ __global__ void k_voronoi(Dims d, int ns, const int* __restrict__ sz, const int*
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int z = 0, y = 0, x = 0;
     if (i < d.N) decode(d, i, z, y, x);
-    long long bd = 0x7fffffffffffffffll;
+    U bd = ~U(0);
     int best = 0;
     for (int base = 0; base < ns; base += 1024) {
         const int cnt = ns - base < 1024 ? ns - base : 1024;
@@ -2114,17 +2118,137 @@ __global__ void k_voronoi(Dims d, int ns, const int* __restrict__ sz, const int*
         }
         __syncthreads();
         for (int t = 0; t < cnt; ++t) {
-            long long a = z - lz[t]; a = a < 0 ? -a : a;
-            long long b = y - ly[t]; b = b < 0 ? -b : b;
-            long long e = x - lx[t]; e = e < 0 ? -e : e;
+            int a = z - lz[t]; a = a < 0 ? -a : a;
+            int b = y - ly[t]; b = b < 0 ? -b : b;
+            int e = x - lx[t]; e = e < 0 ? -e : e;
             a = a < d.Nz - a ? a : d.Nz - a;
             b = b < d.Ny - b ? b : d.Ny - b;
             e = e < d.Nx - e ? e : d.Nx - e;
-            const long long dd = a * a + b * b + e * e;
+            const U dd = U(a) * U(a) + U(b) * U(b) + U(e) * U(e);
             if (dd < bd) { bd = dd; best = base + t; }
         }
     }
     if (i < d.N) phi[i] = best + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Voronoi connectivity normalisation (demos/gpk.cpp:122-189): keep the
+// largest face-connected component of every label (ties: the component whose
+// first cell in flat order comes first), then attach the other cells layer by
+// layer to the most frequent attached face-neighbour label (ties: the smaller
+// label), taking the orientation of the first face neighbour with that label.
+// ---------------------------------------------------------------------------
+__global__ void k_site_g(int n, const int32_t* __restrict__ phi, const float* __restrict__ sg,
+                         float* __restrict__ g32) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        g32[i] = sg[phi[i] - 1];
+}
+
+__global__ void k_cc_init(int n, int* __restrict__ comp) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) comp[i] = i;
+}
+
+// one propagation round: comp[i] = min over i and its same-label face
+// neighbours, then one pointer jump.  Values only decrease and always name a
+// cell of i's component, so the fixed point is the component's first cell.
+__global__ void k_cc_step(Dims d, int dim3, const int32_t* __restrict__ phi, int* __restrict__ comp,
+                          int* __restrict__ changed) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < d.N; i += gridDim.x * blockDim.x) {
+        int f[6];
+        faces(d, i, f);
+        const int32_t p = phi[i];
+        const int c0 = __hip_atomic_load(comp + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int m = c0;
+        const int nf = dim3 ? 6 : 4;
+        for (int k = 0; k < nf; ++k) {
+            if (phi[f[k]] != p) continue;
+            const int cj = __hip_atomic_load(comp + f[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            m = cj < m ? cj : m;
+        }
+        const int cm = __hip_atomic_load(comp + m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        m = cm < m ? cm : m;
+        if (m < c0 && atomicMin(comp + i, m) > m) atomicOr(changed, 1);
+    }
+}
+
+__global__ void k_cc_size(int n, const int* __restrict__ comp, int* __restrict__ csz) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(csz + comp[i], 1);
+}
+
+// best[label] = max over the label's components of (size, -first cell)
+__global__ void k_cc_best(int n, const int32_t* __restrict__ phi, const int* __restrict__ comp,
+                          const int* __restrict__ csz, unsigned long long* __restrict__ best) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (comp[i] == i)
+            atomicMax(best + phi[i], ((unsigned long long)(uint32_t)csz[i] << 32) | (0xFFFFFFFFull - uint32_t(i)));
+}
+
+__global__ void k_cc_attached(int n, const int32_t* __restrict__ phi, const int* __restrict__ comp,
+                              const unsigned long long* __restrict__ best, uint8_t* __restrict__ att,
+                              uint8_t* __restrict__ detached) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t root = 0xFFFFFFFFu - uint32_t(best[phi[i]] & 0xFFFFFFFFull);
+        const bool a = uint32_t(comp[i]) == root;
+        att[i] = a ? 1 : 0;
+        detached[i] = a ? 0 : 1;
+    }
+}
+
+// one attachment layer over the detached list, from the state before the layer
+__global__ void k_attach_scan(Dims d, int dim3, int nu, const int* __restrict__ ul,
+                              const uint8_t* __restrict__ att, const int32_t* __restrict__ phi,
+                              const float* __restrict__ g32, int32_t* __restrict__ tg,
+                              float* __restrict__ og, uint8_t* __restrict__ ok) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const int cc = ul[u];
+    ok[u] = 0;
+    if (att[cc]) return;
+    int f[6];
+    faces(d, cc, f);
+    const int nf = dim3 ? 6 : 4;
+    int32_t cand[6];
+    int sup[6], src[6], nc = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (k >= nf || !att[f[k]]) continue;
+        const int32_t v = phi[f[k]];
+        int q = nc;
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+            if (t < nc && cand[t] == v && q == nc) q = t;
+        if (q == nc) { cand[nc] = v; src[nc] = f[k]; sup[nc] = 0; ++nc; }
+#pragma unroll
+        for (int t = 0; t < 6; ++t)
+            if (t == q) ++sup[t];
+    }
+    if (nc == 0) return;
+    int b = 0;
+#pragma unroll
+    for (int q = 1; q < 6; ++q)
+        if (q < nc && (sup[q] > sup[b] || (sup[q] == sup[b] && cand[q] < cand[b]))) b = q;
+    tg[u] = cand[b];
+    og[u] = g32[src[b]];
+    ok[u] = 1;
+}
+
+__global__ void k_attach_apply(int nu, const int* __restrict__ ul, const int32_t* __restrict__ tg,
+                               const float* __restrict__ og, const uint8_t* __restrict__ ok,
+                               int32_t* __restrict__ phi, float* __restrict__ g32,
+                               uint8_t* __restrict__ att, int* __restrict__ applied) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu || !ok[u]) return;
+    const int cc = ul[u];
+    phi[cc] = tg[u];
+    g32[cc] = og[u];
+    att[cc] = 1;
+    atomicAdd(applied, 1);
+}
+
+template <class R>
+__global__ void k_widen_g(int n, const float* __restrict__ g32, R* __restrict__ g) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) g[i] = R(g32[i]);
 }
 
 } // namespace gpkd

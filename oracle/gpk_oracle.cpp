@@ -1311,7 +1311,8 @@ struct Grid {
 // ---------------------------------------------------------------------------
 // Voronoi initial condition (demos/gpk.cpp:51-189), brute force.
 // ---------------------------------------------------------------------------
-int voronoi(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, float* g) {
+int voronoi(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, float* g,
+            int64_t* detached = nullptr) {
     const int N = Nz * Ny * Nx;
     if (n_seeds <= 0 || n_seeds > N) return GPK_E_INVALID;
     std::mt19937 rng(seed);
@@ -1379,6 +1380,7 @@ int voronoi(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, fl
     size_t natt = 0;
     for (int i = 0; i < N; ++i)
         if (largest[comp_label[comp_of[i]]] == comp_of[i]) { att[i] = 1; ++natt; }
+    if (detached) *detached = int64_t(N) - int64_t(natt);
     while (natt < static_cast<size_t>(N)) {
         std::vector<int> cells; std::vector<int32_t> tg; std::vector<float> og;
         for (int cc = 0; cc < N; ++cc) {
@@ -1548,6 +1550,15 @@ void og_stats_get(void* h, og_stats* o) {
 }
 int og_voronoi(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed, int32_t* phi, float* g) {
     return voronoi(Nz, Ny, Nx, n_seeds, seed, phi, g);
+}
+// cells outside the largest face component of their nearest-site label,
+// i.e. the cells the normalisation re-attaches
+int64_t og_voronoi_detached(int Nz, int Ny, int Nx, int n_seeds, uint32_t seed) {
+    std::vector<int32_t> phi(size_t(Nz) * Ny * Nx);
+    std::vector<float> g(phi.size());
+    int64_t det = -1;
+    if (voronoi(Nz, Ny, Nx, n_seeds, seed, phi.data(), g.data(), &det) != 0) return -1;
+    return det;
 }
 // float GaussKernel weights, for closed-form checks (sum = 1 etc.)
 int og_gauss(float sigma, float* w, int cap, int* radius, float* w0, float* w1, float* w2, int* rg) {

@@ -70,6 +70,8 @@ def oracle_lib():
         L.og_n_grains.argtypes = [vp]
         L.og_solute_sum.restype = C.c_double
         L.og_solute_sum.argtypes = [vp]
+        L.og_voronoi_detached.restype = C.c_int64
+        L.og_voronoi_detached.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32]
         L.og_stats_get.argtypes = [vp, C.POINTER(OgStats)]
         L.og_voronoi.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, vp, vp]
         L.og_gauss.argtypes = [C.c_float, vp, C.c_int, C.POINTER(C.c_int), vp, vp, vp,
@@ -286,3 +288,9 @@ def gauss(sigma):
     n = 2 * r.value + 1
     m = 2 * rg.value + 1
     return w[:n].copy(), r.value, w0[:m].copy(), w1[:m].copy(), w2[:m].copy(), rg.value
+
+
+def voronoi_detached(Nz, Ny, Nx, n_seeds, seed):
+    """Cells of the nearest-site map outside their label's largest face
+    component (the cells voronoi_init re-attaches)."""
+    return int(oracle_lib().og_voronoi_detached(Nz, Ny, Nx, n_seeds, seed))

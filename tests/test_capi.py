@@ -78,6 +78,17 @@ def test_voronoi_host_matches_oracle():
         assert np.array_equal(g, go), dims
 
 
+@pytest.mark.parametrize("dims,ng,seed", [((1, 96, 96), 200, 42), ((24, 24, 24), 60, 42),
+                                          ((40, 36, 32), 300, 5), ((1, 64, 64), 1500, 3),
+                                          ((20, 20, 20), 900, 7), ((16, 24, 20), 700, 1)])
+def test_voronoi_cases_have_detached_cells(dims, ng, seed):
+    # the device Voronoi tests use these cases: each needs re-attachment
+    assert O.voronoi_detached(*dims, ng, seed) > 0
+    phi, g = gpk_amd.voronoi_host(*dims, ng, seed)
+    po, go = O.voronoi(*dims, ng, seed)
+    assert np.array_equal(phi.ravel(), po) and np.array_equal(g.ravel(), go)
+
+
 def test_voronoi_rejects_bad_seed_count():
     with pytest.raises(gpk_amd.GPKError):
         gpk_amd.voronoi_host(1, 8, 8, 0, 1)

@@ -68,8 +68,14 @@ def test_topology_simple_move_matches_oracle():
             assert o.simple_move(int(cell), nl) == G.topology_simple_move(int(cell), nl), (cell, nl)
 
 
-@pytest.mark.parametrize("dims,ng,seed", [((1, 96, 96), 200, 42), ((24, 24, 24), 60, 42),
-                                          ((40, 36, 32), 300, 5)])
+# every case has cells outside their label's largest face component, so the
+# device connectivity normalisation re-attaches cells in each of them
+# (test_capi.test_voronoi_cases_have_detached_cells)
+VORONOI_CASES = [((1, 96, 96), 200, 42), ((24, 24, 24), 60, 42), ((40, 36, 32), 300, 5),
+                 ((1, 64, 64), 1500, 3), ((20, 20, 20), 900, 7), ((16, 24, 20), 700, 1)]
+
+
+@pytest.mark.parametrize("dims,ng,seed", VORONOI_CASES)
 def test_device_voronoi_matches_oracle(dims, ng, seed):
     G = gpk_amd.GPKGrid(*dims, 1.0, _abi.isotropic(1.0, 1.0))
     G.voronoi_init(ng, seed)
