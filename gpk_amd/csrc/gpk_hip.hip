@@ -282,8 +282,9 @@ struct Engine final : EngineBase {
     // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
     // flags / list of the interface labels, hipcub scratch for the label list
     bool broad = false;
-    DevBuf kw, lflag, llist;
+    DevBuf kw, lflag, llist, blidx, bbb, bboxes, bsx, bsy, bsz;
     std::vector<int32_t> hlabels;
+    std::vector<BroadBox> hboxes;
     DevBuf scal, dsum, tagctr, minmax, noise_amp;
     DevBuf cub_tmp;
     size_t cub_bytes = 0;
@@ -343,7 +344,7 @@ struct Engine final : EngineBase {
                                &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
                                &rs_gain, &rs_best, &rs_valid, &rs_dirty,
                                &eps_b, &eps_c,
-                               &llist};
+                               &llist, &blidx, &bbb, &bboxes, &bsx, &bsy, &bsz};
         for (auto* b : all) t += b->bytes;
         return t;
     }
@@ -654,14 +655,79 @@ struct Engine final : EngineBase {
             HIPCHK(hipStreamSynchronize(stream));
         }
         const int kr = W.radius;
-        const int gb = gsblk(N, 256);
-        for (const int32_t p : hlabels) {
-            k_broad_x<R><<<gb, 256, 0, stream>>>(dims, p, kr, kw.as<R>(), phi.as<int32_t>(), work2.as<R>());
-            k_broad_yz<R, 1><<<gb, 256, 0, stream>>>(dims, kr, kw.as<R>(), work2.as<R>(), work3.as<R>());
-            k_broad_yz<R, 2><<<gb, 256, 0, stream>>>(dims, kr, kw.as<R>(), work3.as<R>(), work4.as<R>());
-            k_broad_fd<R><<<nblk(nb, 256), 256, 0, stream>>>(dims, p, nb, bandA.as<int>(), phi.as<int32_t>(),
-                                                            work4.as<R>(), dx_f, W.sigma_cells, nx.as<R>(),
-                                                            ny.as<R>(), nz.as<R>(), kappa.as<R>());
+        if (ng > 0) {
+            if (blidx.bytes < size_t(L) * 4) blidx.alloc(size_t(L) * 4);
+            if (bbb.bytes < size_t(ng) * 24) bbb.alloc(size_t(ng) * 24);
+            k_broad_label_index<<<nblk(ng, 256), 256, 0, stream>>>(ng, llist.as<int>(), blidx.as<int>());
+            k_broad_bbox_init<<<nblk(6 * ng, 256), 256, 0, stream>>>(ng, bbb.as<int>());
+            k_broad_bbox<<<nblk(nb, 256), 256, 0, stream>>>(dims, nb, bandA.as<int>(), phi.as<int32_t>(),
+                                                            blidx.as<int>(), bbb.as<int>());
+            std::vector<int> hb(size_t(ng) * 6);
+            HIPCHK(hipMemcpyAsync(hb.data(), bbb.p, hb.size() * 4, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            // boxes: interface bounding box + 2 (difference stencil), widened by
+            // kr for the y and x passes; a side reaching its period is whole
+            struct Iv { int s, n; };
+            auto box = [](int lo, int hi, int n) { return hi - lo + 1 >= n ? Iv{0, n} : Iv{lo, hi - lo + 1}; };
+            auto widen = [](Iv v, int k, int n) { return v.n + 2 * k >= n ? Iv{0, n} : Iv{v.s - k, v.n + 2 * k}; };
+            hboxes.resize(ng);
+            std::vector<long long> sz(size_t(ng) * 3);
+            for (int t = 0; t < ng; ++t) {
+                const int* b = &hb[size_t(t) * 6];
+                const Iv Bz = box(b[0] - 2, b[1] + 2, Nz), By = box(b[2] - 2, b[3] + 2, Ny),
+                         Bx = box(b[4] - 2, b[5] + 2, Nx);
+                const Iv Wz = widen(Bz, kr, Nz), Wy = widen(By, kr, Ny);
+                BroadBox& X = hboxes[t];
+                X.label = hlabels[t];
+                X.zs[0] = Bz.s; X.zn[0] = Bz.n; X.zs[1] = Wz.s; X.zn[1] = Wz.n;
+                X.ys[0] = By.s; X.yn[0] = By.n; X.ys[1] = Wy.s; X.yn[1] = Wy.n;
+                X.xs = Bx.s; X.xn = Bx.n;
+                sz[size_t(t) * 3 + 0] = (long long)Bz.n * By.n * Bx.n;
+                sz[size_t(t) * 3 + 1] = (long long)Wz.n * By.n * Bx.n;
+                sz[size_t(t) * 3 + 2] = (long long)Wz.n * Wy.n * Bx.n;
+            }
+            // chunks of consecutive labels whose x-pass boxes fit the scratch cap;
+            // offsets are relative to the chunk
+            const long long cap = std::max<long long>(N, 1LL << 24);
+            std::vector<int> cut{0};
+            std::vector<long long> tot;   // per chunk: z, y, x totals
+            long long o0 = 0, o1 = 0, o2 = 0;
+            for (int t = 0; t < ng; ++t) {
+                if (t > cut.back() && o2 + sz[size_t(t) * 3 + 2] > cap) {
+                    tot.insert(tot.end(), {o0, o1, o2});
+                    cut.push_back(t);
+                    o0 = o1 = o2 = 0;
+                }
+                BroadBox& X = hboxes[t];
+                X.off[0] = o0; X.off[1] = o1; X.off[2] = o2;
+                o0 += sz[size_t(t) * 3 + 0]; o1 += sz[size_t(t) * 3 + 1]; o2 += sz[size_t(t) * 3 + 2];
+            }
+            tot.insert(tot.end(), {o0, o1, o2});
+            cut.push_back(ng);
+            long long m0 = 0, m1 = 0, m2 = 0;
+            for (size_t c = 0; c + 1 < cut.size(); ++c) {
+                m0 = std::max(m0, tot[3 * c]); m1 = std::max(m1, tot[3 * c + 1]); m2 = std::max(m2, tot[3 * c + 2]);
+            }
+            if (bsz.bytes < size_t(m0) * sizeof(R)) bsz.alloc(size_t(m0) * sizeof(R));
+            if (bsy.bytes < size_t(m1) * sizeof(R)) bsy.alloc(size_t(m1) * sizeof(R));
+            if (bsx.bytes < size_t(m2) * sizeof(R)) bsx.alloc(size_t(m2) * sizeof(R));
+            if (bboxes.bytes < size_t(ng) * sizeof(BroadBox)) bboxes.alloc(size_t(ng) * sizeof(BroadBox));
+            HIPCHK(hipMemcpyAsync(bboxes.p, hboxes.data(), size_t(ng) * sizeof(BroadBox), hipMemcpyHostToDevice,
+                                  stream));
+            const BroadBox* dbx = bboxes.as<BroadBox>();
+            auto grid_of = [](long long n) { return int(std::max(1LL, std::min<long long>((n + 255) / 256, 65536))); };
+            for (size_t c = 0; c + 1 < cut.size(); ++c) {
+                const int c0 = cut[c], nbox = cut[c + 1] - cut[c];
+                k_broad_box_x<R><<<grid_of(tot[3 * c + 2]), 256, 0, stream>>>(
+                    dims, kr, nbox, dbx + c0, tot[3 * c + 2], kw.as<R>(), phi.as<int32_t>(), bsx.as<R>());
+                k_broad_box_y<R><<<grid_of(tot[3 * c + 1]), 256, 0, stream>>>(
+                    dims, kr, nbox, dbx + c0, tot[3 * c + 1], kw.as<R>(), bsx.as<R>(), bsy.as<R>());
+                k_broad_box_z<R><<<grid_of(tot[3 * c]), 256, 0, stream>>>(
+                    dims, kr, nbox, dbx + c0, tot[3 * c], kw.as<R>(), bsy.as<R>(), bsz.as<R>());
+                k_broad_box_fd<R><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, nb, bandA.as<int>(), phi.as<int32_t>(), blidx.as<int>(), c0, cut[c + 1], dbx,
+                    bsz.as<R>(), dx_f, W.sigma_cells, nx.as<R>(), ny.as<R>(), nz.as<R>(), kappa.as<R>());
+            }
         }
         k_broad_finish<R><<<nblk(nb, 256), 256, 0, stream>>>(
             dims, model, nb, bandA.as<int>(), phi.as<int32_t>(), g.as<R>(), c.as<R>(), T.as<R>(), rho.as<R>(), E_el.as<R>(), nx.as<R>(),

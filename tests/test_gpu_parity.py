@@ -90,6 +90,17 @@ def test_broad_kernel_geometry_3d(mode):
 
 
 @pytest.mark.parametrize("mode", ["strict32", "fp64"])
+def test_broad_kernel_partial_boxes_3d(mode):
+    # sigma 4.6 cells (radius 17, geometry radius 19 > 18) on 64 x 80 x 96 with
+    # 30 grains: most labels' smoothing boxes are strictly smaller than the
+    # grid along y and x and some wrap across the periodic faces, so the
+    # box-restricted passes are exercised beyond the whole-grid case
+    phi, g = O.voronoi(64, 80, 96, 30, 11)
+    o, G = pair((64, 80, 96), _abi.isotropic(1.0, 1.0), 0.0, 4.6, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 6, check_every=2)
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp64"])
 def test_recrystallisation_2d(mode):
     o, G = pair((1, 80, 80), _abi.recrystallisation(1e-9, 0.5, 2e6), 0.0, 1.5, mode, {"phi": rex_setup()},
                 dx=1e-6)
