@@ -269,7 +269,7 @@ struct Engine final : EngineBase {
     DevBuf Pk, Ps;
     DevBuf gain_s, pcell, kgain, kgain2, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
         pflag2, pidx;
-    DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt;
+    DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt, pkey, pvfull;
     bool mask_fresh = false;    // mask matches the current labels
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
@@ -339,7 +339,7 @@ struct Engine final : EngineBase {
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
                                &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain, &kgain2,
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
-                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &cmp_cnt, &count, &unres, &ulist,
+                               &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &cmp_cnt, &pkey, &pvfull, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
                                &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
                                &rs_gain, &rs_best, &rs_valid, &rs_dirty,
@@ -844,6 +844,19 @@ struct Engine final : EngineBase {
                     slot.as<int>(), projw.as<double>(), vraw.as<R>(), vn.as<R>(),
                     reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
             });
+        } else if (nb > 0 && nlab <= 32768) {
+            if (pkey.bytes < size_t(N) * 4) { pkey.alloc(size_t(N) * 4); pvfull.alloc(size_t(N) * sizeof(R)); }
+            k_proj_keys<R><<<gsblk(N, 256), 256, 0, stream>>>(N, mask.as<uint8_t>(), phi.as<int32_t>(),
+                                                              phi_comp.as<int32_t>(), slot.as<int>(), vraw.as<R>(),
+                                                              pkey.as<uint32_t>(), pvfull.as<R>());
+            if (dim3)
+                k_project_keyed<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, proj_r, nb, bandA.as<int>(), pkey.as<uint32_t>(), pvfull.as<R>(), vraw.as<R>(),
+                    projw.as<double>(), vn.as<R>(), reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
+            else
+                k_project_keyed<R, 2><<<nblk(nb, 256), 256, 0, stream>>>(
+                    dims, proj_r, nb, bandA.as<int>(), pkey.as<uint32_t>(), pvfull.as<R>(), vraw.as<R>(),
+                    projw.as<double>(), vn.as<R>(), reinterpret_cast<typename UBits<R>::T*>(&ds()->vmax_bits));
         } else if (nb > 0) {
             if (dim3)
                 k_project<R, 3><<<nblk(nb, 256), 256, 0, stream>>>(

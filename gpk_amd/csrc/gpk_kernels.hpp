@@ -614,6 +614,77 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     }
 }
 
+// ---------------------------------------------------------------------------
+// Projection for radii other than the tiled one (3-D sigma above 2.27 gives
+// r = 4..9): full-grid pair keys and velocities are written once per step,
+// then each band cell walks its window rows branch-free in the reference
+// order.  Same sums as project_value (labels < 2^15: the key packs the pair).
+// ---------------------------------------------------------------------------
+template <class R>
+__global__ void k_proj_keys(int n, const uint8_t* __restrict__ mask, const int32_t* __restrict__ phi,
+                            const int32_t* __restrict__ phi_comp, const int* __restrict__ slot,
+                            const R* __restrict__ vraw, uint32_t* __restrict__ key, R* __restrict__ vfull) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        uint32_t k = 0xFFFFFFFFu;
+        R v = R(0);
+        if (mask[j]) {
+            const uint32_t p = uint32_t(phi[j]), q = uint32_t(phi_comp[j]);
+            if (p != q) {
+                const uint32_t lo = p < q ? p : q, hi = p < q ? q : p;
+                k = (((lo << 16) | hi) << 1) | (p == lo ? 0u : 1u);
+                v = vraw[slot[j]];
+            }
+        }
+        key[j] = k;
+        vfull[j] = v;
+    }
+}
+
+template <class R, int DIM>
+__global__ void __launch_bounds__(256)
+k_project_keyed(Dims d, int r, int nband, const int* __restrict__ band, const uint32_t* __restrict__ key,
+                const R* __restrict__ vfull, const R* __restrict__ vraw, const double* __restrict__ wtab,
+                R* __restrict__ vout, typename UBits<R>::T* __restrict__ vmax_bits) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= nband) return;
+    const int i = band[a];
+    const uint32_t ki = key[i];
+    R out = vraw[a];
+    if (ki != 0xFFFFFFFFu) {
+        const uint32_t pair = ki >> 1;
+        const R oi = (ki & 1u) ? R(-1) : R(1);
+        int z0, y0, x0;
+        decode(d, i, z0, y0, x0);
+        double num = 0.0, den = 0.0;
+        const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+        const int xstart = wrapi(x0 - r, d.Nx);
+        for (int dz = dzb; dz <= dze; ++dz) {
+            const int z = wrapi(z0 + dz, d.Nz);
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = wrapi(y0 + dy, d.Ny);
+                const uint32_t* krow = key + lin(d, z, y, 0);
+                const R* vrow = vfull + lin(d, z, y, 0);
+                const double* wr = wtab + dy * dy + dz * dz;
+                int x = xstart;
+                for (int ddx = -r; ddx <= r; ++ddx) {
+                    const uint32_t kj = krow[x];
+                    const bool hit = (kj >> 1) == pair;
+                    const double w = wr[ddx * ddx];
+                    // (-w) v == -(w v) exactly; a miss adds +0.0
+                    const double t = __longlong_as_double(__double_as_longlong(w * double(vrow[x])) ^
+                                                          (static_cast<long long>(kj & 1u) << 63));
+                    num += keep_if(hit, t);
+                    den += keep_if(hit, w);
+                    if (++x == d.Nx) x = 0;
+                }
+            }
+        }
+        if (den > 0.0) out = oi * R(num / den);
+    }
+    vout[a] = out;
+    atomic_max_abs(vmax_bits, out);
+}
+
 // resident / best-candidate Gaussian scores of the balanced remap
 // (src/gpk_transitions.cpp:53-91) from LDS
 // Remap-score cache: a band cell's scores (gain, best label) depend only on
