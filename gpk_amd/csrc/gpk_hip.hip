@@ -271,6 +271,7 @@ struct Engine final : EngineBase {
         pflag2, pidx;
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt, pkey, pvfull;
     bool mask_fresh = false;    // mask matches the current labels
+    int flip_batch = 4, pair_batch = 4;   // rounds launched before the first host check
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
@@ -1064,8 +1065,11 @@ struct Engine final : EngineBase {
             fc.qflags = qflags.as<int>(); fc.s = ds();
             int round = 0;
             const int bfs_grid = std::max(1, std::min(m, 512));
+            // the first batch is as long as the last step's transaction: rounds
+            // past the end are cheap no-ops, host checks are not
+            int batch = flip_batch;
             while (true) {
-                for (int r = 0; r < 4; ++r, ++round) {
+                for (int r = 0; r < batch; ++r, ++round) {
                     k_flip_round<<<nblk(m, 256), 256, 0, stream>>>(fc, round);
                     k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, round, fbq.as<int>());
                     k_flip_bfs_big<<<1, 1024, 0, stream>>>(fc, fbq.as<int>(), mark.as<int>(),
@@ -1076,7 +1080,9 @@ struct Engine final : EngineBase {
                 if (hs->undecided <= 0) break;
                 if (round > 4 * (m + 8))
                     throw GpkError(GPK_E_HIP, "flip transaction did not converge");
+                batch = 4;
             }
+            flip_batch = round > flip_batch ? round : std::max(4, flip_batch - 2);
         }
         mask_fresh = false;
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
@@ -1168,15 +1174,17 @@ struct Engine final : EngineBase {
                 pc.g = g.p; pc.Pg = Pg.p; pc.count = count.as<int>(); pc.s = ds();
                 if (pdeps.bytes < size_t(npairs) * PAIR_DEPS * 4) pdeps.alloc(size_t(npairs) * PAIR_DEPS * 4);
                 k_pair_deps<<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>());
-                int rounds = 0;
+                int rounds = 0, batch = pair_batch;
                 while (true) {
-                    for (int r = 0; r < 4; ++r, ++rounds)
+                    for (int r = 0; r < batch; ++r, ++rounds)
                         k_pair_round<R><<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>());
                     pull_scalars();
                     if (hs->undecided <= 0) break;
                     if (rounds > 4 * (npairs + 8))
                         throw GpkError(GPK_E_HIP, "remap transaction did not converge");
+                    batch = 4;
                 }
+                pair_batch = rounds > pair_batch ? rounds : std::max(4, pair_batch - 1);
                 mask_fresh = false;
                 k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
                     npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
