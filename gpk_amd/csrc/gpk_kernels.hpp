@@ -196,9 +196,16 @@ struct TiledLds {
 
 // stencil sums of one interface cell from its window row masks
 // (rowmask(jz, jy): bit jx set when window position jx carries the resident)
+// stencil sums of one interface cell from its window row masks
+// (rowmask(jz, jy): bit jx set when window position jx carries the resident),
+// or of two cells at z0 and z0 + 1 with the same resident (3-D, `pair`): the
+// y sums of a window plane are then formed once for both cells -- plane jz of
+// the upper cell is plane jz + 1 of the lower -- and each cell adds its own
+// planes in its own window order, so every value is the one the cell alone
+// would get.
 template <class R, int DIM, int RG, int LAW, class RowMask>
 __device__ __forceinline__ void tiled_cell(
-    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0,
+    const Dims& d, const GeomParams& gp, const gpk_model_desc& model, int x0, int y0, int z0, bool pair,
     int32_t p, const R* lut, const R* __restrict__ w0, const R* __restrict__ w1,
     const R* __restrict__ w2, RowMask rowmask,
     const int32_t* __restrict__ phi, const int* __restrict__ slot, const R* __restrict__ g,
@@ -207,12 +214,13 @@ __device__ __forceinline__ void tiled_cell(
     R* __restrict__ onz, R* __restrict__ okappa, R* __restrict__ vraw,
     int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
-    const int i = lin(d, z0, y0, x0);
     R Ix = 0, Iy = 0, Iz = 0, Ixx = 0, Iyy = 0, Izz = 0, Ixy = 0, Ixz = 0, Iyz = 0;
-    const int nzw = DIM == 3 ? S::W : 1;
+    R Jx = 0, Jy = 0, Jz = 0, Jxx = 0, Jyy = 0, Jzz = 0, Jxy = 0, Jxz = 0, Jyz = 0;   // upper cell
+    // one extra plane when any lane of the wavefront carries a pair
+    const int nzw = DIM == 3 ? S::W + (__ballot(pair) != 0ull ? 1 : 0) : 1;
     for (int jz = 0; jz < nzw; ++jz) {
         R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
-#pragma unroll 4
+#pragma unroll 8
         for (int jy = 0; jy < S::W; ++jy) {
             const unsigned long long m = rowmask(jz, jy);
             R R0, R1, R2;
@@ -240,12 +248,22 @@ __device__ __forceinline__ void tiled_cell(
             S11 = fmar(gy1, R1, S11); S20 = fmar(gy2, R0, S20);
         }
         if (DIM == 3) {
-            const R gz0 = w0[jz], gz1 = w1[jz], gz2 = w2[jz];
-            Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
-            Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
-            Ixy = fmar(gz0, S11, Ixy);
-            Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
-            Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
+            if (jz < S::W) {
+                const R gz0 = w0[jz], gz1 = w1[jz], gz2 = w2[jz];
+                Ix = fmar(gz0, S01, Ix); Iy = fmar(gz0, S10, Iy);
+                Ixx = fmar(gz0, S02, Ixx); Iyy = fmar(gz0, S20, Iyy);
+                Ixy = fmar(gz0, S11, Ixy);
+                Iz = fmar(gz1, S00, Iz); Izz = fmar(gz2, S00, Izz);
+                Ixz = fmar(gz1, S01, Ixz); Iyz = fmar(gz1, S10, Iyz);
+            }
+            if (jz >= 1) {
+                const R gz0 = w0[jz - 1], gz1 = w1[jz - 1], gz2 = w2[jz - 1];
+                Jx = fmar(gz0, S01, Jx); Jy = fmar(gz0, S10, Jy);
+                Jxx = fmar(gz0, S02, Jxx); Jyy = fmar(gz0, S20, Jyy);
+                Jxy = fmar(gz0, S11, Jxy);
+                Jz = fmar(gz1, S00, Jz); Jzz = fmar(gz2, S00, Jzz);
+                Jxz = fmar(gz1, S01, Jxz); Jyz = fmar(gz1, S10, Jyz);
+            }
         } else {
             Ix = S01; Iy = S10; Ixx = S02; Iyy = S20; Ixy = S11;
         }
@@ -254,9 +272,15 @@ __device__ __forceinline__ void tiled_cell(
         (void)t;
         return phi[lin(d, zz, yy, xx)];
     };
-    finish_cell<R, DIM>(d, gp, model, i, slot[i], z0, y0, x0, p, Ix, Iy, Iz, Ixx, Iyy, Izz, Ixy, Ixz,
-                        Iyz, cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp,
-                        LawTag<LAW>{});
+    for (int h = 0; h < (pair ? 2 : 1); ++h) {
+        const int zc = z0 + h;
+        const int i = lin(d, zc, y0, x0);
+        const bool u = h != 0;
+        finish_cell<R, DIM>(d, gp, model, i, slot[i], zc, y0, x0, p, u ? Jx : Ix, u ? Jy : Iy, u ? Jz : Iz,
+                            u ? Jxx : Ixx, u ? Jyy : Iyy, u ? Jzz : Izz, u ? Jxy : Ixy, u ? Jxz : Ixz,
+                            u ? Jyz : Iyz, cube_at, phi, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw,
+                            phi_comp, g_comp, LawTag<LAW>{});
+    }
 }
 
 // LAW = 1: compiled for the plain isotropic law only (LawTag<1>).  At radius 8
@@ -347,20 +371,46 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
             if (hkey[h] == -1) break;
             h = (h + 1) & (GEO_HCAP - 1);
         }
-        cres[q] = r;
-        atomicAdd(&bcount[r == 0xFFFFu ? S::MAXB : r / S::MAXRES], 1);
+        // 3-D: a cell at even lz takes its band neighbour at lz + 1 when both
+        // carry the same label (bit 8: head of a pair, bit 9: taken by one)
+        unsigned short fl = 0;
+        if (DIM == 3 && r != 0xFFFFu) {
+            const int zg = z0t + lz;
+            if ((lz & 1) == 0) {
+                if (lz + 1 < S::TZ && zg + 1 < d.Nz) {
+                    const int ip = lin(d, zg + 1, y0t + ly, x0t + lx);
+                    if (mask[ip] && phi[ip] == p) fl = 0x100;
+                }
+            } else {
+                const int ip = lin(d, zg - 1, y0t + ly, x0t + lx);
+                if (mask[ip] && phi[ip] == p) fl = 0x200;
+            }
+        }
+        cres[q] = r == 0xFFFFu ? r : (unsigned short)(r | fl);
+        if (!(fl & 0x200)) atomicAdd(&bcount[r == 0xFFFFu ? S::MAXB : r / S::MAXRES], 1);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         int o = 0;
-        for (int b = 0; b <= S::MAXB; ++b) { boff[b] = o; o += bcount[b]; bcount[b] = 0; }
+        for (int b = 0; b <= S::MAXB; ++b) { boff[b] = o; o += bcount[b]; }
     }
+    // the hash is no longer read: its slots count each batch's pairs (from
+    // the front of the batch) and single cells (from the back), so that
+    // wavefronts run pairs or singles but rarely both
+    static_assert(128 + S::MAXB < GEO_HCAP, "batch counters must fit the hash");
+    int* const fcnt = hkey;
+    int* const bcnt = hkey + 128;
+    for (int e = threadIdx.x; e <= S::MAXB; e += blockDim.x) { fcnt[e] = 0; bcnt[e] = 0; }
     __syncthreads();
     for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const uint32_t r = cres[q];
+        const uint32_t cr = cres[q];
+        if (cr != 0xFFFFu && (cr & 0x200u)) continue;     // handled with its lz - 1 neighbour
+        const uint32_t r = cr == 0xFFFFu ? cr : (cr & 0xFFu);
+        const uint32_t pr = cr == 0xFFFFu ? 0u : ((cr >> 8) & 1u);
         const int b = r == 0xFFFFu ? S::MAXB : int(r) / S::MAXRES;
         const uint32_t t = r == 0xFFFFu ? 0u : r % S::MAXRES;
-        blist[boff[b] + atomicAdd(&bcount[b], 1)] = uint16_t((t << 12) | uint32_t(list[q]));
+        const int pos = pr ? boff[b] + atomicAdd(&fcnt[b], 1) : boff[b] + bcount[b] - 1 - atomicAdd(&bcnt[b], 1);
+        blist[pos] = uint16_t((pr << 14) | (t << 12) | uint32_t(list[q]));
     }
     const int nr = nres;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
@@ -395,12 +445,16 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         for (int u = boff[bb] + threadIdx.x; u < bend; u += blockDim.x) {
             const uint32_t pk = blist[u];
             const int e = int(pk & 0xFFFu);
-            const int t = int(pk >> 12);
+            const int t = int((pk >> 12) & 3u);
+            const bool pr = (pk >> 14) & 1u;
             const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
             const unsigned long long* rows = rm + (lz * S::SY + ly) * S::MAXRES + t;
-            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, resl[b0 + t], lut, w0, w1, w2,
+            tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, pr, resl[b0 + t], lut, w0, w1, w2,
                                    [&](int jz, int jy) {
-                                       return rows[(jz * S::SY + jy) * S::MAXRES] >> lx;
+                                       // the extra plane of a pair; clamped for the
+                                       // single cells of the same wavefront (unused)
+                                       const int jc = jz < S::SZ - lz ? jz : S::SZ - 1 - lz;
+                                       return rows[(jc * S::SY + jy) * S::MAXRES] >> lx;
                                    },
                                    phi, slot, g, c, T, rho, E_el, onx, ony, onz, okappa, vraw, phi_comp, g_comp);
         }
@@ -412,7 +466,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
         const int x0 = x0t + lx, y0 = y0t + ly, z0 = z0t + lz;
         const int32_t p = phi[lin(d, z0, y0, x0)];
-        tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0, y0, z0, p, lut, w0, w1, w2,
+        tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0, y0, z0, false, p, lut, w0, w1, w2,
                                [&](int jz, int jy) {
                                    const int z = DIM == 3 ? wrapi(z0 + jz - RG, d.Nz) : z0;
                                    const int y = wrapi(y0 + jy - RG, d.Ny);
