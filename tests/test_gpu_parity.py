@@ -44,7 +44,7 @@ def pair(dims, model, D, sigma, mode, init, dx=1.0):
     return o, G
 
 
-def test_config1_grain_growth_2d_strict32_100_steps():
+def test_grain_growth_2d_256_strict32_100_steps():
     # 256^2, 200 Voronoi grains (seed 42), sigma 1.5, dt at the stability limit
     phi, g = O.voronoi(1, 256, 256, 200, 42)
     o, G = pair((1, 256, 256), _abi.isotropic(1.0, 1.0), 0.0, 1.5, "strict32", {"phi": phi, "g": g})
@@ -52,7 +52,7 @@ def test_config1_grain_growth_2d_strict32_100_steps():
 
 
 @pytest.mark.parametrize("mode", ["fp32", "fp64"])
-def test_config1_grain_growth_2d_fast_modes(mode):
+def test_grain_growth_2d_256_fast_modes(mode):
     phi, g = O.voronoi(1, 256, 256, 200, 42)
     o, G = pair((1, 256, 256), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
     lockstep(o, G, 60, check_every=5)

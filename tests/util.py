@@ -1,4 +1,4 @@
-"""Shared set-ups for the parity tests (initial conditions of the configs)."""
+"""Shared set-ups for the parity tests (initial conditions of the benchmark and deck workloads)."""
 import os
 
 import numpy as np
