@@ -384,7 +384,7 @@ struct Engine final : EngineBase {
         Pa.alloc(n * 4); Pb.alloc(n * 4); Plo.alloc(n * 4); Phi.alloc(n * 4); pstate.alloc(n * 4);
         Pg.alloc(2 * n * rs);
         ulist.alloc(n * 4); utgt.alloc(n * 4); utgg.alloc(n * rs); uflag.alloc(n);
-        scal.alloc(sizeof(Scalars)); dsum.alloc(((n + 1023) / 1024 + 16) * 8);
+        scal.alloc(sizeof(Scalars)); dsum.alloc(((n + 1023) / 1024 + (n + 1048575) / 1048576 + 16) * 8);
         tagctr.alloc(16); minmax.alloc(16); noise_amp.alloc(16);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(Scalars), hipHostMallocDefault));
         std::memset(hs, 0, sizeof(Scalars));
@@ -798,6 +798,15 @@ struct Engine final : EngineBase {
     }
 
     int sweep_rho() const { return (model.flags & GPK_FLAG_DISLOCATION) ? 1 : 0; }
+    // remap-score cache invalidation around relabelled cells
+    void mark_dirty(int n, int ncell, const int* c0, const int* c1, const int* st) {
+        if (n <= 0) return;
+        const long long W = 2 * remap_r + 1;
+        const long long total = (long long)n * ncell * (dim3 ? W * W * W : W * W);
+        const int grid = int(std::max(1LL, std::min<long long>((total + 255) / 256, 65536)));
+        k_mark_dirty<<<grid, 256, 0, stream>>>(dims, dim3, remap_r, n, ncell, c0, c1, st,
+                                                 rs_dirty.as<uint8_t>());
+    }
     // the FP32 mode decides flips near the threshold again in FP64; strict32
     // keeps the reference's float decision
     static constexpr bool guard_band() { return sizeof(R) == 4 && !kDirect; }
@@ -1088,8 +1097,8 @@ struct Engine final : EngineBase {
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
             phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
-            network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho(), dims, dim3, remap_r,
-            rs_dirty.as<uint8_t>());
+            network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho());
+        mark_dirty(m, 1, Ccell.as<int>(), nullptr, network ? state.as<int>() : nullptr);
         return m;
     }
 
@@ -1189,8 +1198,8 @@ struct Engine final : EngineBase {
                 k_pair_apply<R><<<nblk(npairs, 256), 256, 0, stream>>>(
                     npairs, Pa.as<int>(), Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(),
                     Pg.as<R>(), phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), pid.as<int>(), ds(),
-                    rho.as<R>(), R(model.rho_rex), sweep_rho(), dims, dim3, remap_r,
-                    rs_dirty.as<uint8_t>());
+                    rho.as<R>(), R(model.rho_rex), sweep_rho());
+                mark_dirty(npairs, 2, Pa.as<int>(), Pb.as<int>(), pstate.as<int>());
             }
         }
         if (!mask_fresh) update_mask();
@@ -1279,14 +1288,25 @@ struct Engine final : EngineBase {
         return static_cast<float>(interface_cells()) / static_cast<float>(N);
     }
 
+    // second and third levels of the canonical FP64 sum over the chunk sums
+    // at dsum[0, nch); blocks until the value is on the host
+    double sum_groups(int nch) {
+        const int ng = (nch + 1023) / 1024;
+        double* p1 = dsum.as<double>();
+        k_sum_groups<<<ng, 256, 0, stream>>>(nch, p1, p1 + nch);
+        k_sum_tail<<<1, 1, 0, stream>>>(ng, p1 + nch, p1 + nch + ng);
+        double v = 0.0;
+        HIPCHK(hipMemcpyAsync(&v, p1 + nch + ng, 8, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return v;
+    }
+
     double phase_volume(int32_t label) override {
         const int nch = (N + 1023) / 1024;
-        k_phase_terms<R><<<nblk(nch, 256), 256, 0, stream>>>(N, label, phi.as<int32_t>(),
+        k_phase_terms<R><<<nblk(nch, 2), 256, 0, stream>>>(N, label, phi.as<int32_t>(),
                                                              phi_comp.as<int32_t>(), mask.as<uint8_t>(),
                                                              alpha.as<R>(), dsum.as<double>());
-        k_sum_final<<<1, 1, 0, stream>>>(nch, dsum.as<double>(), dsum.as<double>() + nch);
-        double v = 0.0;
-        HIPCHK(hipMemcpyAsync(&v, dsum.as<double>() + nch, 8, hipMemcpyDeviceToHost, stream));
+        const double v = sum_groups(nch);
         HIPCHK(hipStreamSynchronize(stream));
         return v;
     }
@@ -1307,10 +1327,8 @@ struct Engine final : EngineBase {
 
     double solute_sum() override {
         const int nch = (N + 1023) / 1024;
-        k_sum_chunks<R><<<nblk(nch, 256), 256, 0, stream>>>(N, c.as<R>(), dsum.as<double>());
-        k_sum_final<<<1, 1, 0, stream>>>(nch, dsum.as<double>(), dsum.as<double>() + nch);
-        double v = 0.0;
-        HIPCHK(hipMemcpyAsync(&v, dsum.as<double>() + nch, 8, hipMemcpyDeviceToHost, stream));
+        k_sum_chunks<R><<<nblk(nch, 4), 256, 0, stream>>>(N, c.as<R>(), dsum.as<double>());
+        const double v = sum_groups(nch);
         HIPCHK(hipStreamSynchronize(stream));
         return v;
     }

@@ -1464,8 +1464,7 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
                              const R* __restrict__ Canew, const int* __restrict__ state,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
-                             int all_accept, R* __restrict__ rho, R rho_rex, int sweep, Dims dd,
-                             int dim3, int rr, uint8_t* __restrict__ dirty) {
+                             int all_accept, R* __restrict__ rho, R rho_rex, int sweep) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool acc = false, def = false;
     if (k < m) {
@@ -1476,7 +1475,6 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
             g[cell] = Cg[k];
             alpha[cell] = R(1);
             if (sweep) rho[cell] = rho_rex;   // src/gpk_transitions.cpp:308
-            mark_dirty_box(dd, dim3, cell, rr, dirty);
             atomicSub(count + Cold[k], 1);
             atomicAdd(count + Cnew[k], 1);
             acc = true;
@@ -1818,7 +1816,7 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
                              const int* __restrict__ pstate, const R* __restrict__ Pg,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ pid, Scalars* s, R* __restrict__ rho, R rho_rex,
-                             int sweep, Dims dd, int dim3, int rr, uint8_t* __restrict__ dirty) {
+                             int sweep) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const int a = Pa[q], b = Pb[q];
@@ -1828,9 +1826,31 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
     g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
     alpha[a] = R(1); alpha[b] = R(1);
     if (sweep) { rho[a] = rho_rex; rho[b] = rho_rex; }   // src/gpk_transitions.cpp:173
-    mark_dirty_box(dd, dim3, a, rr, dirty);
-    mark_dirty_box(dd, dim3, b, rr, dirty);
     atomicAdd(&s->exchanges, 2);
+}
+
+// remap-score cache: the (2r+1)^d box around every relabelled cell is marked
+// dirty, one thread per (entry, cell of the entry, box position); entry k
+// relabelled its cells when st[k] == ACCEPT (st null: every entry did).
+// Flips have one cell per entry (c0), pairs two (c0, c1).
+__global__ void k_mark_dirty(Dims d, int dim3, int r, int n, int ncell, const int* __restrict__ c0,
+                             const int* __restrict__ c1, const int* __restrict__ st,
+                             uint8_t* __restrict__ dirty) {
+    const int W = 2 * r + 1;
+    const long long box = dim3 ? (long long)W * W * W : (long long)W * W;
+    const long long total = (long long)n * ncell * box;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long e = t / box;
+        const int o = int(t - e * box);
+        const int k = int(e / ncell);
+        if (st && st[k] != ACCEPT) continue;
+        const int cell = (e - (long long)k * ncell) == 0 ? c0[k] : c1[k];
+        int z0, y0, x0;
+        decode(d, cell, z0, y0, x0);
+        const int dz = dim3 ? o / (W * W) - r : 0, dy = (o / W) % W - r, dx = o % W - r;
+        dirty[lin(d, wrapi(z0 + dz, d.Nz), wrapi(y0 + dy, d.Ny), wrapi(x0 + dx, d.Nx))] = 1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2008,6 +2028,13 @@ __global__ void k_reject_gather(Dims d, int dim3, const int32_t* __restrict__ ph
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.N; j += gridDim.x * blockDim.x) {
         int nbj[6];
         const int nn = part_faces(d, dim3, j, nbj);
+        // no rejecting source around j (the common case): the delta is +0
+        bool any = q[j] > R(0);
+        for (int a = 0; a < nn; ++a) any |= q[nbj[a]] > R(0);
+        if (!any) {
+            c[j] += R(0);
+            continue;
+        }
         int src[7];
         int ns = 0;
         src[ns++] = j;
@@ -2165,44 +2192,84 @@ __global__ void k_scatter_band(int nband, const int* __restrict__ band, const R*
     if (a < nband) dst[band[a]] = src[a];
 }
 
-// FP64 sum in the canonical chunk order (1024 cells, then 1024 chunks)
+// FP64 sum in the canonical chunk order: sequential over each 1024-cell
+// chunk, then over each group of 1024 chunk sums, then over the groups.  Each
+// level stages its operands in LDS with coalesced loads and one thread per
+// chunk (group) adds them in order.
 template <class R>
-__global__ void k_sum_chunks(int n, const R* __restrict__ v, double* __restrict__ out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_sum_chunks(int n, const R* __restrict__ v, double* __restrict__ out) {
+    __shared__ double st[4][1025];          // 4 chunks per block; the pad spreads the rows over banks
     const int nch = (n + 1023) / 1024;
-    if (b >= nch) return;
-    double s = 0.0;
-    const int e = (b + 1) * 1024 < n ? (b + 1) * 1024 : n;
-    for (int i = b * 1024; i < e; ++i) s += double(v[i]);
-    out[b] = s;
+    const int ch0 = blockIdx.x * 4;
+    for (int t = threadIdx.x; t < 4096; t += blockDim.x) {
+        const int k = t >> 10;
+        const long long i = (long long)(ch0 + k) * 1024 + (t & 1023);
+        st[k][t & 1023] = i < n ? double(v[i]) : 0.0;
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < 4 && ch0 + k < nch) {
+        const int cnt = n - (ch0 + k) * 1024 < 1024 ? n - (ch0 + k) * 1024 : 1024;
+        double s = 0.0;
+        for (int i = 0; i < cnt; ++i) s += st[k][i];
+        out[ch0 + k] = s;
+    }
 }
-__global__ void k_sum_final(int nch, const double* __restrict__ p1, double* __restrict__ out) {
+
+// group sums: block g adds chunk sums [1024 g, 1024 g + 1024) in order
+__global__ void __launch_bounds__(256) k_sum_groups(int nch, const double* __restrict__ p1,
+                                                    double* __restrict__ gsum) {
+    __shared__ double st[1024];
+    const int b = blockIdx.x * 1024;
+    const int cnt = nch - b < 1024 ? nch - b : 1024;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) st[t] = p1[b + t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < cnt; ++i) s += st[i];
+        gsum[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_sum_tail(int ng, const double* __restrict__ gsum, double* __restrict__ out) {
     if (threadIdx.x || blockIdx.x) return;
     double tot = 0.0;
-    for (int b = 0; b < nch; b += 1024) {
-        double s = 0.0;
-        const int e = b + 1024 < nch ? b + 1024 : nch;
-        for (int i = b; i < e; ++i) s += p1[i];
-        tot += s;
-    }
+    for (int g = 0; g < ng; ++g) tot += gsum[g];
     *out = tot;
 }
 
 // phase volume terms per cell (src/gpk_simulation.cpp:78-86), chunk-summed
 template <class R>
-__global__ void k_phase_terms(int N, int32_t label, const int32_t* __restrict__ phi,
-                              const int32_t* __restrict__ phi_comp, const uint8_t* __restrict__ mask,
-                              const R* __restrict__ alpha, double* __restrict__ out) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256)
+k_phase_terms(int N, int32_t label, const int32_t* __restrict__ phi,
+              const int32_t* __restrict__ phi_comp, const uint8_t* __restrict__ mask,
+              const R* __restrict__ alpha, double* __restrict__ out) {
+    // two chunks per block; per cell the two terms in order, a term that does
+    // not apply staged as +0.0 (the partial sums are never -0, so adding it
+    // changes nothing)
+    __shared__ double st[2][2049];
     const int nch = (N + 1023) / 1024;
-    if (b >= nch) return;
-    double s = 0.0;
-    const int e = (b + 1) * 1024 < N ? (b + 1) * 1024 : N;
-    for (int i = b * 1024; i < e; ++i) {
-        if (phi[i] == label) s += double(alpha[i]);
-        if (mask[i] && phi_comp[i] == label && phi_comp[i] != phi[i]) s += 1.0 - double(alpha[i]);
+    const int ch0 = blockIdx.x * 2;
+    for (int t = threadIdx.x; t < 2048; t += blockDim.x) {
+        const int k = t >> 10;
+        const long long i = (long long)(ch0 + k) * 1024 + (t & 1023);
+        double t1 = 0.0, t2 = 0.0;
+        if (i < N) {
+            const double a = double(alpha[i]);
+            if (phi[i] == label) t1 = a;
+            if (mask[i] && phi_comp[i] == label && phi_comp[i] != phi[i]) t2 = 1.0 - a;
+        }
+        st[k][2 * (t & 1023)] = t1;
+        st[k][2 * (t & 1023) + 1] = t2;
     }
-    out[b] = s;
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < 2 && ch0 + k < nch) {
+        const int cnt = N - (ch0 + k) * 1024 < 1024 ? N - (ch0 + k) * 1024 : 1024;
+        double s = 0.0;
+        for (int i = 0; i < 2 * cnt; ++i) s += st[k][i];
+        out[ch0 + k] = s;
+    }
 }
 
 __global__ void k_count_u8(int N, const uint8_t* __restrict__ m, int* __restrict__ out) {
