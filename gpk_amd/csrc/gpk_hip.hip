@@ -985,8 +985,13 @@ struct Engine final : EngineBase {
         set_int(&ds()->guarded, 0);
         int m = 0;
         if (nb > 0) m = flips(nb);
-        pull_scalars();
-        const int accepted = hs->accepted;
+        // without the topology test every candidate is accepted
+        // (src/gpk_transitions.cpp:299-303): the count is known here
+        int accepted = m;
+        if (model.flags & GPK_FLAG_GRAIN_NETWORK) {
+            pull_scalars();
+            accepted = hs->accepted;
+        }
         int exchanges = 0;
         if (accepted > 0) {
             exchanges = remap_sweep();
@@ -1203,6 +1208,7 @@ struct Engine final : EngineBase {
             }
         }
         if (!mask_fresh) update_mask();
+        if (npairs == 0) return 0;
         pull_scalars();
         return hs->exchanges;
     }
