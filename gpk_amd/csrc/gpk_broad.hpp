@@ -116,18 +116,43 @@ __global__ void k_broad_bbox_init(int ng, int* __restrict__ bb) {
 }
 
 // bounding box of each listed label's interface cells: bb[6 t .. 6 t + 5] =
-// zmin, zmax, ymin, ymax, xmin, xmax
+// zmin, zmax, ymin, ymax, xmin, xmax.  Consecutive band cells mostly share a
+// label, so each wavefront reduces per label before its atomics.
+__device__ __forceinline__ int wave_min_i32(int v) {
+    for (int o = 32; o > 0; o >>= 1) { const int u = __shfl_xor(v, o); v = u < v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+    for (int o = 32; o > 0; o >>= 1) { const int u = __shfl_xor(v, o); v = u > v ? u : v; }
+    return v;
+}
+
 __global__ void k_broad_bbox(Dims d, int nb, const int* __restrict__ band, const int32_t* __restrict__ phi,
                              const int* __restrict__ lidx, int* __restrict__ bb) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nb) return;
-    const int i = band[a];
-    int z, y, x;
-    decode(d, i, z, y, x);
-    int* b = bb + 6 * lidx[phi[i]];
-    atomicMin(b + 0, z); atomicMax(b + 1, z);
-    atomicMin(b + 2, y); atomicMax(b + 3, y);
-    atomicMin(b + 4, x); atomicMax(b + 5, x);
+    const int lane = threadIdx.x & 63;
+    int t = -1, z = 0, y = 0, x = 0;
+    if (a < nb) {
+        const int i = band[a];
+        decode(d, i, z, y, x);
+        t = lidx[phi[i]];
+    }
+    unsigned long long todo = __ballot(t >= 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const int tt = __shfl(t, leader);
+        const bool mine = t == tt;
+        const int zmin = wave_min_i32(mine ? z : 0x7fffffff), zmax = wave_max_i32(mine ? z : -1);
+        const int ymin = wave_min_i32(mine ? y : 0x7fffffff), ymax = wave_max_i32(mine ? y : -1);
+        const int xmin = wave_min_i32(mine ? x : 0x7fffffff), xmax = wave_max_i32(mine ? x : -1);
+        if (lane == leader) {
+            int* b = bb + 6 * tt;
+            atomicMin(b + 0, zmin); atomicMax(b + 1, zmax);
+            atomicMin(b + 2, ymin); atomicMax(b + 3, ymax);
+            atomicMin(b + 4, xmin); atomicMax(b + 5, xmax);
+        }
+        todo &= ~__ballot(mine);
+    }
 }
 
 template <class R>
