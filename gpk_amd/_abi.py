@@ -78,7 +78,8 @@ class BatchStats(C.Structure):
     """gpk_batch_stats (include/gpk_c.h)."""
     _fields_ = [("band_cells", C.c_int64), ("accepted", C.c_int64), ("exchanges", C.c_int64),
                 ("proposals", C.c_int64), ("instances", C.c_int32), ("workgroup_size", C.c_int32),
-                ("workgroups", C.c_int32), ("guarded", C.c_int64)]
+                ("workgroups", C.c_int32), ("guarded", C.c_int64), ("deferred", C.c_int64),
+                ("eroded", C.c_int64)]
 
 
 class StepStats(C.Structure):

@@ -17,9 +17,13 @@
 //   (src/gpk_physics.cpp:152-193), dt (src/gpk_simulation.cpp:11-40),
 //   alpha and flips (src/gpk_transitions.cpp:251-313), balanced Gaussian
 //   remap (38-178), time advance.
-// Scope: 2-D instances; models without the grain-network topology test
-// (every flip candidate is accepted, src/gpk_transitions.cpp:299-303),
-// D = 0, no noise; labels below 1024 per instance.
+// Models with the grain-network topology test decide their flip candidates
+// one after another in index order (src/gpk_transitions.cpp:288-313): each
+// candidate's 3x3 topological numbers, label count and, for a label with
+// several branches, a periodic face-connectivity search run by the whole
+// workgroup; every 8th step erodes unresolved grains (180-249).
+// Scope: 2-D instances, D = 0, no noise, no dislocation or elastic terms;
+// labels below 1024 per instance.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,6 +51,8 @@ struct Inst {
 struct InstStat {
     double time, dt;
     long long band_total, accepted_total, exchanges_total, proposals_total, guarded_total;
+    long long deferred_total, eroded_total;
+    long long step_id;    // steps taken by the instance
     int band, accepted, exchanges, proposals;
     long long t0, t1;     // wall clock at the step's start and end
 };
@@ -69,6 +75,8 @@ struct BatchArgs {
     float dx_f;
     R dt_req, cap;
     int proj_r, remap_r;
+    int network;          // grain-network topology test on flips
+    int min_cells;        // grains below this size are unresolved
     const R *w0, *w1, *w2, *tab;
     RowLutDesc lutd;
     const double *projw, *remapw;
@@ -108,6 +116,11 @@ struct Shared {
     double rbuf[WG / 64];
     int hist[BATCH_LMAX];
     int inst;
+    // connectivity search state
+    int bfs_cnt[2];
+    uint32_t bfs_present[2];
+    uint32_t bfs_adj;
+    int bfs_res, bfs_nb;
 };
 
 // exclusive prefix of a 0/1 flag over the workgroup, in thread order
@@ -187,6 +200,133 @@ __device__ int band_compact(Shared<WG>& sh, const Dims& d, const int32_t* __rest
     return nb;
 }
 
+// Are the face branches of label L around `cell` (the cell itself left out
+// when excl) one connected component of L on the periodic instance?  The
+// reference answers with a depth-first search from the first branch towards
+// the others (src/gpk_connectivity.cpp:79-115).  Here every branch grows at
+// once, one breadth-first level per pass of the workgroup; colours that meet
+// are joined, and the search ends when all colours are joined (connected) or
+// a joined group has no frontier left (its component is closed without the
+// others: disconnected) -- the same answer.  mark[c] = (tag << 3) | colour
+// for cells visited by search `tag`; the frontiers hold (cell << 3) | colour
+// and never exceed the instance size, as every cell is claimed once.
+template <int WG>
+__device__ bool batch_connected(Shared<WG>& sh, const Dims& d, const int32_t* __restrict__ phi, int cell,
+                                int32_t L, bool excl, int* __restrict__ mark, int* __restrict__ fr0,
+                                int* __restrict__ fr1, int tag) {
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        int f[6];
+        faces(d, cell, f);
+        int nb = 0;
+        for (int q = 0; q < 4; ++q) {
+            if (phi[f[q]] != L) continue;
+            __hip_atomic_store(mark + f[q], (tag << 3) | nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            fr0[nb] = (f[q] << 3) | nb;
+            ++nb;
+        }
+        sh.bfs_nb = nb;
+        sh.bfs_cnt[0] = nb;
+        sh.bfs_present[0] = (1u << nb) - 1u;
+        sh.bfs_adj = 0u;
+        sh.bfs_res = nb <= 1 ? 1 : -1;
+    }
+    __syncthreads();
+    int cur = 0;
+    while (true) {
+        if (tid == 0 && sh.bfs_res < 0) {
+            const int nb = sh.bfs_nb;
+            const uint32_t adj = sh.bfs_adj;
+            int comp[4] = {0, 1, 2, 3};
+            bool changed = true;
+            while (changed) {
+                changed = false;
+                for (int a = 0; a < nb; ++a)
+                    for (int b = 0; b < nb; ++b)
+                        if (((adj >> (a * 4 + b)) & 1u) && comp[a] != comp[b]) {
+                            const int m = comp[a] < comp[b] ? comp[a] : comp[b];
+                            comp[a] = m;
+                            comp[b] = m;
+                            changed = true;
+                        }
+            }
+            bool all = true;
+            for (int q = 1; q < nb; ++q) all &= comp[q] == comp[0];
+            if (all) {
+                sh.bfs_res = 1;
+            } else {
+                const uint32_t present = sh.bfs_present[cur];
+                for (int q = 0; q < nb; ++q) {
+                    bool open = false;
+                    for (int p = 0; p < nb; ++p) open |= comp[p] == comp[q] && ((present >> p) & 1u);
+                    if (!open) sh.bfs_res = 0;
+                }
+            }
+        }
+        __syncthreads();
+        if (sh.bfs_res >= 0) break;
+        const int nxt = cur ^ 1;
+        const int* fin = cur ? fr1 : fr0;
+        int* fout = cur ? fr0 : fr1;
+        const int n = sh.bfs_cnt[cur];
+        if (tid == 0) {
+            sh.bfs_cnt[nxt] = 0;
+            sh.bfs_present[nxt] = 0u;
+        }
+        __syncthreads();
+        for (int e = tid; e < n; e += WG) {
+            const int v = fin[e];
+            const int col = v & 7;
+            const int want = (tag << 3) | col;
+            int f[6];
+            faces(d, v >> 3, f);
+            for (int q = 0; q < 4; ++q) {
+                const int c2 = f[q];
+                if ((excl && c2 == cell) || phi[c2] != L) continue;
+                int old = __hip_atomic_load(mark + c2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                while (true) {
+                    if ((old >> 3) == tag) {
+                        const int oc = old & 7;
+                        if (oc != col) atomicOr(&sh.bfs_adj, (1u << (oc * 4 + col)) | (1u << (col * 4 + oc)));
+                        break;
+                    }
+                    const int prev = atomicCAS(mark + c2, old, want);
+                    if (prev == old) {
+                        fout[atomicAdd(&sh.bfs_cnt[nxt], 1)] = (c2 << 3) | col;
+                        atomicOr(&sh.bfs_present[nxt], 1u << col);
+                        break;
+                    }
+                    old = prev;
+                }
+            }
+        }
+        __syncthreads();
+        cur = nxt;
+    }
+    const bool ok = sh.bfs_res == 1;
+    __syncthreads();
+    return ok;
+}
+
+// 3x3 occupancy of labels ol and nl around cell (centre left out), bit
+// (dy + 1) * 3 + (dx + 1)
+__device__ __forceinline__ void cube_masks(const Dims& d, const int32_t* __restrict__ phi, int cell, int32_t ol,
+                                           int32_t nl, uint32_t& co, uint32_t& cn) {
+    const int y0 = cell / d.Nx, x0 = cell - y0 * d.Nx;
+    co = 0u;
+    cn = 0u;
+    for (int dy = -1; dy <= 1; ++dy) {
+        const int y = wrapi(y0 + dy, d.Ny);
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (dy == 0 && dx == 0) continue;
+            const int32_t l = phi[y * d.Nx + wrapi(x0 + dx, d.Nx)];
+            const int t = (dy + 1) * 3 + dx + 1;
+            co |= l == ol ? 1u << t : 0u;
+            cn |= l == nl ? 1u << t : 0u;
+        }
+    }
+}
+
 template <class R, int WG, bool kDirect>
 __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
     const Inst I = A.inst[b];
@@ -234,28 +374,98 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
     const R stable = R(Eps<R>::big) < dti ? R(Eps<R>::big) : dti;
     R dt = A.dt_req < R(0) ? stable : (stable < A.dt_req ? stable : A.dt_req);
     dt = A.cap < dt ? A.cap : dt;
-    // alpha and flips: without the topology test every candidate is accepted
-    // (src/gpk_transitions.cpp:251-313)
+    // alpha and flips (src/gpk_transitions.cpp:251-313)
     // (FP32 mode: decisions within the guard band taken again in FP64)
     const R idt = R(double(dt)) / dx;
-    int acc = 0, gd = 0;
-    for (int a = tid; a < nb; a += WG) {
-        const int i = band[a];
-        R an;
-        bool guarded;
-        const bool below = alpha_step<R>(alpha[i], vn[a], idt, double(dt), double(dx), !kDirect, an, guarded);
-        gd += guarded ? 1 : 0;
-        if (below && phi_comp[i] != phi[i]) {
-            phi[i] = phi_comp[i];
-            g[i] = g_comp[i];
-            alpha[i] = R(1);
-            ++acc;
-        } else {
-            alpha[i] = an;
+    int acc = 0, gd = 0, deferred = 0;
+    if (!A.network) {
+        // without the topology test every candidate is accepted
+        for (int a = tid; a < nb; a += WG) {
+            const int i = band[a];
+            R an;
+            bool guarded;
+            const bool below = alpha_step<R>(alpha[i], vn[a], idt, double(dt), double(dx), !kDirect, an, guarded);
+            gd += guarded ? 1 : 0;
+            if (below && phi_comp[i] != phi[i]) {
+                phi[i] = phi_comp[i];
+                g[i] = g_comp[i];
+                alpha[i] = R(1);
+                ++acc;
+            } else {
+                alpha[i] = an;
+            }
+        }
+    } else {
+        // candidates in band (= index) order; alpha_new stored for all
+        int* cand = reinterpret_cast<int*>(props);   // sort scratch: 4 ints per cell
+        int* mark = cand + I.spow;
+        int* fr0 = cand + 2 * I.spow;
+        int* fr1 = cand + 3 * I.spow;
+        int nc = 0;
+        for (int base = 0; base < nb; base += WG) {
+            const int a = base + tid;
+            bool f = false;
+            int i = 0;
+            if (a < nb) {
+                i = band[a];
+                R an;
+                bool guarded;
+                const bool below =
+                    alpha_step<R>(alpha[i], vn[a], idt, double(dt), double(dx), !kDirect, an, guarded);
+                gd += guarded ? 1 : 0;
+                alpha[i] = an;
+                f = below && phi_comp[i] != phi[i];
+            }
+            int tot;
+            const int pos = block_scan_flag<WG>(sh, f, tot);
+            if (f) cand[nc + pos] = i;
+            nc += tot;
+        }
+        // label counts, kept current as flips are accepted
+        for (int l = tid; l < BATCH_LMAX; l += WG) sh.hist[l] = 0;
+        __syncthreads();
+        for (int i = tid; i < d.N; i += WG) atomicAdd(&sh.hist[phi[i]], 1);
+        __syncthreads();
+        // one candidate after another, each on the labels the earlier ones
+        // left; every thread evaluates the same decision so that the
+        // connectivity searches are entered by the whole workgroup
+        int tag = 0;
+        for (int e = 0; e < nc; ++e) {
+            const int i = cand[e];
+            const int32_t ol = phi[i], nl = phi_comp[i];
+            uint32_t co, cn;
+            cube_masks(d, phi, i, ol, nl, co, cn);
+            const int oo = topo_count(co, false), no = topo_count(cn, false);
+            // src/gpk_connectivity.cpp:179-206: a vanishing old label only
+            // when it is the label's last cell, a new label that touches
+            bool ok = no > 0 && !(oo == 0 && sh.hist[ol] > 1);
+            if (ok && (oo > 1 || no > 1) && tag == 0) {
+                for (int c2 = tid; c2 < d.N; c2 += WG) mark[c2] = 0;
+                __syncthreads();
+            }
+            if (ok && oo > 1) ok = batch_connected<WG>(sh, d, phi, i, ol, true, mark, fr0, fr1, ++tag);
+            if (ok && no > 1) ok = batch_connected<WG>(sh, d, phi, i, nl, false, mark, fr0, fr1, ++tag);
+            __syncthreads();
+            if (tid == 0) {
+                if (ok) {
+                    phi[i] = nl;
+                    g[i] = g_comp[i];
+                    alpha[i] = R(1);
+                    --sh.hist[ol];
+                    ++sh.hist[nl];
+                } else {
+                    const R an = alpha[i];
+                    alpha[i] = an < Eps<R>::flip ? Eps<R>::flip : an;
+                }
+            }
+            acc += (ok && tid == 0) ? 1 : 0;
+            deferred += (!ok && tid == 0) ? 1 : 0;
+            __syncthreads();
         }
     }
     const int accepted = block_sum_int<WG>(sh, acc);
     const int nguard = block_sum_int<WG>(sh, gd);
+    const int ndeferred = block_sum_int<WG>(sh, deferred);
 
     // balanced Gaussian remap (src/gpk_transitions.cpp:38-178), one sweep in 2-D
     int exchanges = 0, np = 0;
@@ -359,8 +569,70 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
             }
         }
         __syncthreads();
-        band_compact<WG>(sh, d, phi, mask, band, slot, false);
     }
+    // unresolved-grain extinction, every 8th step (src/gpk_transitions.cpp:180-249)
+    int eroded = 0;
+    if (A.network && (A.st[b].step_id & 7LL) == 0) {
+        for (int l = tid; l < BATCH_LMAX; l += WG) sh.hist[l] = 0;
+        __syncthreads();
+        for (int i = tid; i < d.N; i += WG) atomicAdd(&sh.hist[phi[i]], 1);
+        __syncthreads();
+        int pres = 0, unr = 0;
+        for (int l = tid; l < BATCH_LMAX; l += WG) {
+            const int h = sh.hist[l];
+            pres += h > 0 ? 1 : 0;
+            unr += (h > 0 && h < A.min_cells) ? 1 : 0;
+        }
+        pres = block_sum_int<WG>(sh, pres);
+        unr = block_sum_int<WG>(sh, unr);
+        if (unr > 0 && unr != pres) {
+            // layer by layer: targets from the labels before the layer
+            int* tgt = reinterpret_cast<int*>(props);
+            R* tgg = reinterpret_cast<R*>(tgt + 2 * I.spow);
+            auto unres = [&](int32_t l) { const int h = sh.hist[l]; return h > 0 && h < A.min_cells; };
+            while (true) {
+                for (int i = tid; i < d.N; i += WG) {
+                    int t = -1;
+                    if (unres(phi[i])) {
+                        const int y0 = i / d.Nx, x0 = i - y0 * d.Nx;
+                        int f[6];
+                        faces(d, i, f);
+                        int bs = -1;
+                        for (int k = 0; k < 4; ++k) {
+                            const int32_t q = phi[f[k]];
+                            if (unres(q)) continue;
+                            int sup = 0;
+                            for (int dy = -1; dy <= 1; ++dy) {
+                                const int y = wrapi(y0 + dy, d.Ny);
+                                for (int dx = -1; dx <= 1; ++dx)
+                                    sup += phi[y * d.Nx + wrapi(x0 + dx, d.Nx)] == q ? 1 : 0;
+                            }
+                            if (sup > bs) {
+                                bs = sup;
+                                t = q;
+                                tgg[i] = g[f[k]];
+                            }
+                        }
+                    }
+                    tgt[i] = t;
+                }
+                __syncthreads();
+                int ch = 0;
+                for (int i = tid; i < d.N; i += WG) {
+                    const int t = tgt[i];
+                    if (t < 0) continue;
+                    phi[i] = t;
+                    g[i] = tgg[i];
+                    alpha[i] = R(1);
+                    ++ch;
+                }
+                ch = block_sum_int<WG>(sh, ch);
+                eroded += ch;
+                if (ch == 0) break;
+            }
+        }
+    }
+    if (accepted > 0 || eroded > 0) band_compact<WG>(sh, d, phi, mask, band, slot, false);
     if (tid == 0) {
         InstStat& S = A.st[b];
         S.dt = double(dt);
@@ -374,6 +646,9 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
         S.exchanges_total += exchanges;
         S.proposals_total += np;
         S.guarded_total += nguard;
+        S.deferred_total += ndeferred;
+        S.eroded_total += eroded;
+        S.step_id += 1;
         S.t1 = (long long)wall_clock64();
     }
     __syncthreads();
@@ -576,6 +851,11 @@ struct Batch final : BatchBase {
         A.cap = R(cap);
         A.proj_r = proj_r;
         A.remap_r = remap_r;
+        A.network = (model.flags & GPK_FLAG_GRAIN_NETWORK) ? 1 : 0;
+        {
+            const double sc = std::max(W.sigma_cells, 1.f);
+            A.min_cells = std::max(2, static_cast<int>(std::ceil(M_PI * sc * sc)));
+        }
         A.w0 = wt0.as<R>(); A.w1 = wt1.as<R>(); A.w2 = wt2.as<R>(); A.tab = tab.as<R>();
         A.lutd = lutd;
         A.projw = projw.as<double>(); A.remapw = remapw.as<double>();
@@ -641,6 +921,8 @@ struct Batch final : BatchBase {
             s->exchanges += x.exchanges_total;
             s->proposals += x.proposals_total;
             s->guarded += x.guarded_total;
+            s->deferred += x.deferred_total;
+            s->eroded += x.eroded_total;
         }
         s->instances = n_inst;
         s->workgroup_size = wg;
@@ -728,17 +1010,16 @@ int gpk_batch_create(int n_inst, const int* Ny, const int* Nx, float dx, float D
         *out = nullptr;
         for (int b = 0; b < n_inst; ++b)
             if (Ny[b] < 3 || Nx[b] < 3) throw GpkError(GPK_E_INVALID, "instances need Ny, Nx >= 3");
+        for (int b = 0; b < n_inst; ++b)
+            if ((long long)Ny[b] * Nx[b] >= (1LL << 28))
+                throw GpkError(GPK_E_INVALID, "batched instances hold fewer than 2^28 cells each");
         if (D != 0.f) throw GpkError(GPK_E_UNSUPPORTED_MODEL, "batched instances run with D = 0");
-        if (model->flags & (GPK_FLAG_GRAIN_NETWORK | GPK_FLAG_SOLUTE_REJECTION | GPK_FLAG_STOCHASTIC |
-                            GPK_FLAG_DISLOCATION | GPK_FLAG_ELASTIC) ||
-            model->kind == GPK_MODEL_PRECIPITATION)
+        if (model->flags & (GPK_FLAG_SOLUTE_REJECTION | GPK_FLAG_STOCHASTIC | GPK_FLAG_DISLOCATION |
+                            GPK_FLAG_ELASTIC) ||
+            model->kind == GPK_MODEL_PRECIPITATION || model->kind == GPK_MODEL_DISLOCATION)
             throw GpkError(GPK_E_UNSUPPORTED_MODEL,
-                           "batched instances: models without grain-network topology, solute rejection "
-                           "or noise");
-        if (model->kind == GPK_MODEL_ISOTROPIC || model->kind == GPK_MODEL_ANISO_GROWTH)
-            throw GpkError(GPK_E_UNSUPPORTED_MODEL,
-                           "batched instances: isotropic and anisotropic growth use the grain-network "
-                           "topology test");
+                           "batched instances: models without solute rejection, noise, stored energy "
+                           "or elastic terms");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
             throw GpkError(GPK_E_NO_DEVICE, "gpk: no HIP device visible");

@@ -297,6 +297,8 @@ typedef struct gpk_batch_stats {
     int32_t instances;
     int32_t workgroup_size, workgroups;   /* launch shape in use */
     int64_t guarded;      /* FP32 mode: flip decisions taken again in FP64 (guard band) */
+    int64_t deferred;     /* grain-network models: candidates the topology test refused */
+    int64_t eroded;       /* grain-network models: cells of unresolved grains relabelled */
 } gpk_batch_stats;
 
 int gpk_batch_create(int n_inst, const int* Ny, const int* Nx, float dx, float D, float gaussian_sigma,

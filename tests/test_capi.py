@@ -36,7 +36,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_abi.GridDesc) == 8 * 4
     assert C.sizeof(_abi.StepStats) == 6 * 8 + 2 * 8 + 8 + 8
     # four counters, three int32 (padded to 8), the guard-band counter
-    assert C.sizeof(_abi.BatchStats) == 4 * 8 + 3 * 4 + 4 + 8
+    assert C.sizeof(_abi.BatchStats) == 4 * 8 + 3 * 4 + 4 + 3 * 8
 
 
 def test_model_defaults_agree_with_python_mirror():

@@ -1,9 +1,10 @@
 """Batched 2-D instances (one persistent kernel per step over a packed
 instance table) against the oracle stepping each instance on its own, bit for
 bit in every precision mode: labels, alpha, orientation, mask, competitor and
-each instance's time.  Workload: lamellar stripes with seeded one-cell edge
+each instance's time.  Workloads: lamellar stripes with seeded one-cell edge
 perturbations under AnisoOstwaldThermo with c = c_sat (pure capillary flow,
-demos/gpk.cpp:1105-1143), D = 0."""
+demos/gpk.cpp:1105-1143), D = 0; Voronoi grains under isotropic growth with
+the grain-network topology test."""
 import numpy as np
 import pytest
 
@@ -90,8 +91,54 @@ def test_batch_launch_shapes_agree_and_autotune_restores_state():
     B.close()
 
 
+GRAIN_CASES = [(40, 48, 40), (56, 56, 70), (64, 40, 50), (33, 71, 45), (48, 48, 90)]
+
+
+@pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
+def test_batch_grain_network_matches_oracle(mode):
+    """Isotropic grain growth (grain-network topology test) from Voronoi
+    grains: flips refused by the 3x3 count or a periodic connectivity search,
+    and unresolved grains eroded every 8th step, all as in the oracle."""
+    shapes = [(ny, nx) for ny, nx, _ in GRAIN_CASES]
+    phis, gs = [], []
+    for k, (ny, nx, ns) in enumerate(GRAIN_CASES):
+        p, g = O.voronoi(1, ny, nx, ns, 11 + k)
+        phis.append(p.reshape(ny, nx))
+        gs.append(g.reshape(ny, nx))
+    steps = 40
+    thermo = _abi.isotropic(1.0, 1.0)
+    B = GPKBatch(shapes, 1.0, thermo, 0.0, 1.5, precision=mode)
+    B.set("phi", phis)
+    B.set("g", gs)
+    B.set("alpha", [np.ones(s, np.float32) for s in shapes])
+    print("batch grain network", mode, GRAIN_CASES, flush=True)
+    B.step(steps)
+    t = B.instances()[0]
+    got = {f: B.get(f) for f in ("phi", "alpha", "g", "mask", "phi_comp")}
+    st = B.stats()
+    tot = {"accepted": 0, "deferred": 0, "eroded": 0, "exchanges": 0}
+    for b, (shape, phi) in enumerate(zip(shapes, phis)):
+        o = O.OracleGrid(1, shape[0], shape[1], 1.0, thermo, 0.0, 1.5, mode)
+        o.set("phi", phi)
+        o.set("g", gs[b].astype(o.real))
+        for _ in range(steps):
+            o.step()
+            ost = o.stats()
+            for key in tot:
+                tot[key] += ost[key]
+        for f in ("phi", "alpha", "g", "mask", "phi_comp"):
+            ref = o.get(f)
+            assert np.array_equal(got[f][b], ref), (b, f, int(np.count_nonzero(got[f][b] != ref)))
+        assert t[b] == o.time, (b, t[b], o.time)
+    # every path of the decision ran: refusals, erosion and remap exchanges
+    assert tot["deferred"] > 0 and tot["eroded"] > 0 and tot["exchanges"] > 0
+    for key in tot:
+        assert st[key] == tot[key], (key, st[key], tot[key])
+    B.close()
+
+
 def test_batch_rejects_unsupported_models():
     with pytest.raises(Exception):
-        GPKBatch([(32, 32)], 1.0, _abi.isotropic(1.0, 1.0), 0.0, 1.5, precision="fp32")
+        GPKBatch([(32, 32)], 1.0, _abi.dislocation(1.0, 1.0, 1.0, 1.0), 0.0, 1.5, precision="fp32")
     with pytest.raises(Exception):
         GPKBatch([(32, 32)], 1.0, model(), 1e-9, 1.5, precision="fp32")
