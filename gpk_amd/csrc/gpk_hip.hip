@@ -468,6 +468,17 @@ struct Engine final : EngineBase {
     void set_int(int* dev, int v) {
         k_fill_i32<<<1, 1, 0, stream>>>(1, dev, v);
     }
+    // several device ints set by one launch
+    void set_ints(std::initializer_list<std::pair<int*, int>> l) {
+        IntSets a{};
+        for (const auto& e : l) {
+            if (a.n == IntSets::CAP) throw GpkError(GPK_E_INVALID, "set_ints: too many values");
+            a.p[a.n] = e.first;
+            a.v[a.n] = e.second;
+            ++a.n;
+        }
+        k_set_ints<<<1, 64, 0, stream>>>(a);
+    }
 
     void rebuild_counts() {
         fill<int32_t>(count, nlab, 0);
@@ -764,12 +775,13 @@ struct Engine final : EngineBase {
 
     // ---- ordered compaction of a byte-flag array (count, scan, scatter);
     // leaves the count in the device scalars (nsel) without reading it back
-    void compact_async(const uint8_t* flags, long long n, int* out) {
+    // (and, given slot_map, slot_map[out[a]] = a)
+    void compact_async(const uint8_t* flags, long long n, int* out, int* slot_map = nullptr) {
         const int nb = int((n + CMP_CHUNK - 1) / CMP_CHUNK);
         if (cmp_cnt.bytes < size_t(nb) * 4) cmp_cnt.alloc(size_t(nb) * 4);
         k_cmp_count<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>());
         k_cmp_scan<<<1, 1024, 0, stream>>>(nb, cmp_cnt.as<int>(), &ds()->nsel);
-        k_cmp_scatter<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>(), out);
+        k_cmp_scatter<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>(), out, slot_map);
     }
 
     // ---- interface mask (src/gpk_geometry.cpp:42-82); mask_fresh records
@@ -782,12 +794,9 @@ struct Engine final : EngineBase {
     // ---- band compaction: mask + sorted interface list (+ slot map) ----
     int band_compact(DevBuf& list, bool with_slot) {
         if (!mask_fresh) update_mask();
-        compact_async(mask.as<uint8_t>(), N, list.as<int>());
+        compact_async(mask.as<uint8_t>(), N, list.as<int>(), with_slot ? slot.as<int>() : nullptr);
         pull_scalars();
-        const int nb = hs->nsel;
-        if (with_slot && nb > 0)
-            k_slot<<<nblk(nb, 256), 256, 0, stream>>>(nb, list.as<int>(), slot.as<int>());
-        return nb;
+        return hs->nsel;
     }
 
     int select_flags(const uint8_t* flags, int n, int* out) {
@@ -978,11 +987,8 @@ struct Engine final : EngineBase {
             k_solid_fraction<R><<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), alpha.as<R>(),
                                                                    work1.as<R>());
         // counters
-        set_int(&ds()->accepted, 0);
-        set_int(&ds()->deferred, 0);
-        set_int(&ds()->exchanges, 0);
-        set_int(&ds()->eroded, 0);
-        set_int(&ds()->guarded, 0);
+        set_ints({{&ds()->accepted, 0}, {&ds()->deferred, 0}, {&ds()->exchanges, 0}, {&ds()->eroded, 0},
+                  {&ds()->guarded, 0}});
         int m = 0;
         if (nb > 0) m = flips(nb);
         // without the topology test every candidate is accepted
@@ -1065,10 +1071,7 @@ struct Engine final : EngineBase {
             k_cand_pos<<<nblk(2 * m, 256), 256, 0, stream>>>(2 * m, Ps.as<unsigned long long>(),
                                                               Cold.as<int>(), pos_old.as<int>(),
                                                               pos_new.as<int>());
-            set_int(&ds()->undecided, m);
-            set_int(&ds()->qcount[0], 0);
-            set_int(&ds()->qcount[1], 0);
-            set_int(&ds()->fbcount, 0);
+            set_ints({{&ds()->undecided, m}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0}});
             FlipCtx fc;
             fc.d = dims; fc.dim3 = dim3; fc.m = m;
             fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
@@ -1085,10 +1088,9 @@ struct Engine final : EngineBase {
             while (true) {
                 for (int r = 0; r < batch; ++r, ++round) {
                     k_flip_round<<<nblk(m, 256), 256, 0, stream>>>(fc, round);
-                    k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, round, fbq.as<int>());
-                    k_flip_bfs_big<<<1, 1024, 0, stream>>>(fc, fbq.as<int>(), mark.as<int>(),
-                                                           reinterpret_cast<int*>(work3.p),
-                                                           reinterpret_cast<int*>(work4.p), tagctr.as<int>());
+                    k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, round, fbq.as<int>(), mark.as<int>(),
+                                                             reinterpret_cast<int*>(work3.p),
+                                                             reinterpret_cast<int*>(work4.p), tagctr.as<int>());
                 }
                 pull_scalars();
                 if (hs->undecided <= 0) break;
@@ -1218,8 +1220,7 @@ struct Engine final : EngineBase {
         const double s = std::max(W.sigma_cells, 1.f);
         const double vol = dim3 ? (4.0 * M_PI * s * s * s / 3.0) : (M_PI * s * s);
         const int min_cells = std::max(2, static_cast<int>(std::ceil(vol)));
-        set_int(&ds()->present, 0);
-        set_int(&ds()->unresolved, 0);
+        set_ints({{&ds()->present, 0}, {&ds()->unresolved, 0}});
         k_label_census<<<nblk(nlab, 256), 256, 0, stream>>>(nlab, count.as<int>(), min_cells,
                                                             unres.as<uint8_t>(), ds());
         pull_scalars();
@@ -1362,10 +1363,7 @@ struct Engine final : EngineBase {
         int po = ol < nl ? 0 : 1, pn = 1 - po;
         HIPCHK(hipMemcpyAsync(pos_old.p, &po, 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(pos_new.p, &pn, 4, hipMemcpyHostToDevice, stream));
-        set_int(&ds()->undecided, 1);
-        set_int(&ds()->qcount[0], 0);
-        set_int(&ds()->qcount[1], 0);
-        set_int(&ds()->fbcount, 0);
+        set_ints({{&ds()->undecided, 1}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0}});
         FlipCtx fc;
         fc.d = dims; fc.dim3 = dim3; fc.m = 1;
         fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
@@ -1374,10 +1372,8 @@ struct Engine final : EngineBase {
         fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
         fc.qflags = qflags.as<int>(); fc.s = ds();
         k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
-        k_flip_bfs<<<1, 256, 0, stream>>>(fc, 0, fbq.as<int>());
-        k_flip_bfs_big<<<1, 1024, 0, stream>>>(fc, fbq.as<int>(), mark.as<int>(),
-                                               reinterpret_cast<int*>(work3.p),
-                                               reinterpret_cast<int*>(work4.p), tagctr.as<int>());
+        k_flip_bfs<<<1, 256, 0, stream>>>(fc, 0, fbq.as<int>(), mark.as<int>(), reinterpret_cast<int*>(work3.p),
+                                          reinterpret_cast<int*>(work4.p), tagctr.as<int>());
         int st = 0;
         HIPCHK(hipMemcpyAsync(&st, state.p, 4, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));

@@ -111,8 +111,10 @@ __global__ void __launch_bounds__(1024) k_cmp_scan(int nb, int* __restrict__ bcn
     if (threadIdx.x == 0) *total = running;
 }
 
+// (slot, when given: slot[out[o]] = o, the inverse map of the list)
 __global__ void __launch_bounds__(CMP_T) k_cmp_scatter(long long n, const uint8_t* __restrict__ f,
-                                                       const int* __restrict__ boff, int* __restrict__ out) {
+                                                       const int* __restrict__ boff, int* __restrict__ out,
+                                                       int* __restrict__ slot) {
     __shared__ int wsum[CMP_T / 64], tot;
     const long long base = (long long)blockIdx.x * CMP_CHUNK + (long long)threadIdx.x * CMP_V;
     uint32_t bits = flag_bits16(f, base, n);
@@ -120,14 +122,22 @@ __global__ void __launch_bounds__(CMP_T) k_cmp_scatter(long long n, const uint8_
     int o = boff[blockIdx.x] + block_incl_scan<CMP_T>(c, wsum, &tot) - c;
     while (bits) {
         const int j = __ffs(bits) - 1;
-        out[o++] = int(base + j);
+        out[o] = int(base + j);
+        if (slot) slot[base + j] = o;
+        ++o;
         bits &= bits - 1u;
     }
 }
 
-__global__ void k_slot(int n, const int* __restrict__ band, int* __restrict__ slot) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < n) slot[band[a]] = a;
+// several device ints set by one launch
+struct IntSets {
+    static constexpr int CAP = 8;
+    int* p[CAP];
+    int v[CAP];
+    int n;
+};
+__global__ void k_set_ints(IntSets a) {
+    if (int(threadIdx.x) < a.n) a.p[threadIdx.x][0] = a.v[threadIdx.x];
 }
 
 template <class R, int DIM, bool kDirect>
@@ -494,6 +504,27 @@ __device__ __forceinline__ void atomic_max_abs(unsigned long long* p, double v) 
 template <class R> struct UBits;
 template <> struct UBits<float> { using T = unsigned int; };
 template <> struct UBits<double> { using T = unsigned long long; };
+__device__ __forceinline__ unsigned int abs_bits(float v) { return __float_as_uint(fabsf(v)); }
+__device__ __forceinline__ unsigned long long abs_bits(double v) {
+    return (unsigned long long)__double_as_longlong(fabs(v));
+}
+
+// Workgroup maximum of |v| bits folded into one atomicMax on dst (the bits
+// of a non-negative float or double order as the values do): wavefront
+// shuffles, one LDS atomic per wavefront, one device atomic per workgroup,
+// instead of one device atomic per wavefront on a single address.  Called by
+// every thread of the workgroup; lds must hold 0 and have been published by
+// a barrier before the call.
+template <class U>
+__device__ __forceinline__ void block_max_to(U v, U& lds, U* __restrict__ dst) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const U u = __shfl_xor(v, o);
+        v = v < u ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(&lds, v);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(dst, lds);
+}
 
 template <class R, int DIM>
 __global__ void __launch_bounds__(256)
@@ -502,11 +533,18 @@ k_project(Dims d, int r, int nband, const int* __restrict__ band, const int* __r
           const int32_t* __restrict__ phi_comp, const double* __restrict__ wtab,
           const R* __restrict__ vraw, R* __restrict__ vout,
           typename UBits<R>::T* __restrict__ vmax_bits) {
+    using U = typename UBits<R>::T;
+    __shared__ U bmax;
+    if (threadIdx.x == 0) bmax = 0;
+    __syncthreads();
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nband) return;
-    const R out = project_value<R, DIM>(d, r, a, band[a], slot, mask, phi, phi_comp, wtab, vraw);
-    vout[a] = out;
-    atomic_max_abs(vmax_bits, out);
+    U mb = 0;
+    if (a < nband) {
+        const R out = project_value<R, DIM>(d, r, a, band[a], slot, mask, phi, phi_comp, wtab, vraw);
+        vout[a] = out;
+        mb = abs_bits(out);
+    }
+    block_max_to<U>(mb, bmax, vmax_bits);
 }
 
 template <class R>
@@ -562,12 +600,16 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     __shared__ short list[H::NT];
     __shared__ int zoff[H::SZ], yoff[H::SY], xoff[H::SX];   // wrapped offsets of the staged box
     __shared__ int nlist;
+    __shared__ typename UBits<R>::T bmax;
     const int ntiles = ntx * nty * ntz;
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;
     const int tx = tile % ntx, ty = (tile / ntx) % nty, tz = tile / (ntx * nty);
     const int x0t = tx * H::TX, y0t = ty * H::TY, z0t = tz * H::TZ;
-    if (threadIdx.x == 0) nlist = 0;
+    if (threadIdx.x == 0) {
+        nlist = 0;
+        bmax = 0;
+    }
     for (int e = threadIdx.x; e < H::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - H::HZ, d.Nz) * d.Ny * d.Nx;
     for (int e = threadIdx.x; e < H::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RP, d.Ny) * d.Nx;
     for (int e = threadIdx.x; e < H::SX; e += blockDim.x) xoff[e] = wrapi(x0t + e - RP, d.Nx);
@@ -621,6 +663,7 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
     }
     __syncthreads();
     const int n = nlist;
+    typename UBits<R>::T mb = 0;
     for (int qq = threadIdx.x; qq < n; qq += blockDim.x) {
         const int e = list[qq];
         const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
@@ -664,8 +707,10 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
         }
         const int i = lin(d, z0t + lz, y0t + ly, x0t + lx);
         vout[slot[i]] = out;
-        atomic_max_abs(vmax_bits, out);
+        const auto b = abs_bits(out);
+        mb = mb < b ? b : mb;
     }
+    block_max_to(mb, bmax, vmax_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -699,44 +744,51 @@ __global__ void __launch_bounds__(256)
 k_project_keyed(Dims d, int r, int nband, const int* __restrict__ band, const uint32_t* __restrict__ key,
                 const R* __restrict__ vfull, const R* __restrict__ vraw, const double* __restrict__ wtab,
                 R* __restrict__ vout, typename UBits<R>::T* __restrict__ vmax_bits) {
+    using U = typename UBits<R>::T;
+    __shared__ U bmax;
+    if (threadIdx.x == 0) bmax = 0;
+    __syncthreads();
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= nband) return;
-    const int i = band[a];
-    const uint32_t ki = key[i];
-    R out = vraw[a];
-    if (ki != 0xFFFFFFFFu) {
-        const uint32_t pair = ki >> 1;
-        const R oi = (ki & 1u) ? R(-1) : R(1);
-        int z0, y0, x0;
-        decode(d, i, z0, y0, x0);
-        double num = 0.0, den = 0.0;
-        const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
-        const int xstart = wrapi(x0 - r, d.Nx);
-        for (int dz = dzb; dz <= dze; ++dz) {
-            const int z = wrapi(z0 + dz, d.Nz);
-            for (int dy = -r; dy <= r; ++dy) {
-                const int y = wrapi(y0 + dy, d.Ny);
-                const uint32_t* krow = key + lin(d, z, y, 0);
-                const R* vrow = vfull + lin(d, z, y, 0);
-                const double* wr = wtab + dy * dy + dz * dz;
-                int x = xstart;
-                for (int ddx = -r; ddx <= r; ++ddx) {
-                    const uint32_t kj = krow[x];
-                    const bool hit = (kj >> 1) == pair;
-                    const double w = wr[ddx * ddx];
-                    // (-w) v == -(w v) exactly; a miss adds +0.0
-                    const double t = __longlong_as_double(__double_as_longlong(w * double(vrow[x])) ^
-                                                          (static_cast<long long>(kj & 1u) << 63));
-                    num += keep_if(hit, t);
-                    den += keep_if(hit, w);
-                    if (++x == d.Nx) x = 0;
+    U mb = 0;
+    if (a < nband) {
+        const int i = band[a];
+        const uint32_t ki = key[i];
+        R out = vraw[a];
+        if (ki != 0xFFFFFFFFu) {
+            const uint32_t pair = ki >> 1;
+            const R oi = (ki & 1u) ? R(-1) : R(1);
+            int z0, y0, x0;
+            decode(d, i, z0, y0, x0);
+            double num = 0.0, den = 0.0;
+            const int dzb = DIM == 3 ? -r : 0, dze = DIM == 3 ? r : 0;
+            const int xstart = wrapi(x0 - r, d.Nx);
+            for (int dz = dzb; dz <= dze; ++dz) {
+                const int z = wrapi(z0 + dz, d.Nz);
+                for (int dy = -r; dy <= r; ++dy) {
+                    const int y = wrapi(y0 + dy, d.Ny);
+                    const uint32_t* krow = key + lin(d, z, y, 0);
+                    const R* vrow = vfull + lin(d, z, y, 0);
+                    const double* wr = wtab + dy * dy + dz * dz;
+                    int x = xstart;
+                    for (int ddx = -r; ddx <= r; ++ddx) {
+                        const uint32_t kj = krow[x];
+                        const bool hit = (kj >> 1) == pair;
+                        const double w = wr[ddx * ddx];
+                        // (-w) v == -(w v) exactly; a miss adds +0.0
+                        const double t = __longlong_as_double(__double_as_longlong(w * double(vrow[x])) ^
+                                                              (static_cast<long long>(kj & 1u) << 63));
+                        num += keep_if(hit, t);
+                        den += keep_if(hit, w);
+                        if (++x == d.Nx) x = 0;
+                    }
                 }
             }
+            if (den > 0.0) out = oi * R(num / den);
         }
-        if (den > 0.0) out = oi * R(num / den);
+        vout[a] = out;
+        mb = abs_bits(out);
     }
-    vout[a] = out;
-    atomic_max_abs(vmax_bits, out);
+    block_max_to<U>(mb, bmax, vmax_bits);
 }
 
 // resident / best-candidate Gaussian scores of the balanced remap
@@ -936,6 +988,7 @@ struct Scalars {
     int present, unresolved;        // label census
     int changed;                    // a component-labelling round lowered some id
     int guarded;                    // flip decisions taken again in FP64 (FP32 mode)
+    int bfs_done;                   // workgroups of k_flip_bfs finished (last one runs the fallback)
 };
 
 template <class R>
@@ -1179,7 +1232,7 @@ struct BfsShared {
     int fcount[2];
     int inserted;
     unsigned long long adj;
-    int present;
+    uint32_t pres[2]; // colours present in each frontier
     int result;       // -1 running, 0 disconnected, 1 connected
     int overflow;
     int nbranch;
@@ -1196,7 +1249,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
     const int nface = c.dim3 ? 6 : 4;
     for (int h = threadIdx.x; h < BFS_HCAP; h += blockDim.x) S.hash[h] = ~0ull;
     if (threadIdx.x == 0) {
-        S.fcount[0] = 0; S.fcount[1] = 0; S.inserted = 0; S.adj = 0ull; S.present = 0;
+        S.fcount[0] = 0; S.fcount[1] = 0; S.inserted = 0; S.adj = 0ull; S.pres[0] = 0u; S.pres[1] = 0u;
         S.result = -1; S.overflow = 0; S.nbranch = 0;
     }
     __syncthreads();
@@ -1215,6 +1268,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
                     S.hash[h] = val; ++S.inserted;
                     S.fr[0][S.fcount[0]] = f[q]; S.frc[0][S.fcount[0]] = uint8_t(col);
                     ++S.fcount[0];
+                    S.pres[0] |= 1u << col;
                     break;
                 }
                 if (int(cur >> 8) == f[q]) {
@@ -1254,8 +1308,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
                 S.result = 1;
             } else {
                 // a merged group with no frontier is closed: disconnected
-                int present = 0;
-                for (int e = 0; e < S.fcount[cur]; ++e) present |= 1 << S.frc[cur][e];
+                const uint32_t present = S.pres[cur];
                 for (int q = 0; q < nb && S.result < 0; ++q) {
                     bool has = false;
                     for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1)) has = true;
@@ -1266,7 +1319,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
         __syncthreads();
         if (S.result != -1) break;
         const int nxt = cur ^ 1;
-        if (threadIdx.x == 0) S.fcount[nxt] = 0;
+        if (threadIdx.x == 0) { S.fcount[nxt] = 0; S.pres[nxt] = 0u; }
         __syncthreads();
         const int nf = S.fcount[cur] < BFS_FCAP ? S.fcount[cur] : BFS_FCAP;
         for (int e = threadIdx.x; e < nf; e += blockDim.x) {
@@ -1289,6 +1342,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
                             S.overflow = 1;
                         } else {
                             S.fr[nxt][pos] = nbc; S.frc[nxt][pos] = uint8_t(col);
+                            atomicOr(&S.pres[nxt], 1u << col);
                         }
                         break;
                     }
@@ -1312,28 +1366,6 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
     return r == -2 ? -1 : r;
 }
 
-__global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, int round, int* __restrict__ fbq) {
-    __shared__ BfsShared S;
-    if (blockIdx.x == 0 && threadIdx.x == 0) c.s->qcount[(round + 1) & 1] = 0;
-    const int n = c.s->qcount[round & 1];
-    for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
-        const int k = c.queue[round & 1][qi];
-        const int flags = c.qflags[k];
-        const int cell = c.Ccell[k];
-        int ok = 1;
-        if (flags & 1) ok = bfs_connected_lds(S, c, cell, c.Cold[k], true, k);
-        if (ok == 1 && (flags & 2)) ok = bfs_connected_lds(S, c, cell, c.Cnew[k], false, k);
-        if (threadIdx.x == 0) {
-            if (ok < 0) {
-                const int p = atomicAdd(&c.s->fbcount, 1);
-                fbq[p] = k;
-            } else {
-                decide(c, k, ok ? ACCEPT : REJECT);
-            }
-        }
-        __syncthreads();
-    }
-}
 
 // Large-component fallback: the same multi-source test with the visited set
 // in device memory and frontiers of grid capacity.  A cell's mark packs the
@@ -1349,17 +1381,17 @@ __device__ __forceinline__ int claim_mark(int* mark, int cell, int tag, int col,
     }
 }
 
-__global__ void __launch_bounds__(1024)
-k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
-               int* __restrict__ fr0, int* __restrict__ fr1, int* __restrict__ tag_ctr) {
+__device__ void flip_bfs_fallback(FlipCtx& c, const int* __restrict__ fbq, int* __restrict__ mark,
+                                  int* __restrict__ fr0, int* __restrict__ fr1, int* __restrict__ tag_ctr) {
     __shared__ int fcount[2];
+    __shared__ uint32_t pres[2];   // colours present in each frontier
     __shared__ unsigned long long adj;
     __shared__ int result, nbranch, tag;
     const Dims& d = c.d;
     const int nface = c.dim3 ? 6 : 4;
-    const int n = c.s->fbcount;
+    const int n = ld_acq(&c.s->fbcount);
     for (int qi = 0; qi < n; ++qi) {
-        const int k = fbq[qi];
+        const int k = ld_acq(fbq + qi);
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
         int ok = 1;
@@ -1371,7 +1403,7 @@ k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
             __syncthreads();
             if (threadIdx.x == 0) {
                 tag = ++(*tag_ctr);
-                fcount[0] = 0; fcount[1] = 0; adj = 0ull; result = -1;
+                fcount[0] = 0; fcount[1] = 0; pres[0] = 0u; pres[1] = 0u; adj = 0ull; result = -1;
                 int f[6];
                 faces(d, cell, f);
                 int nb = 0;
@@ -1379,8 +1411,10 @@ k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
                     if (view_label(f[q], k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
                     const int col = nb++;
                     int oc = 0;
-                    if (claim_mark(mark, f[q], tag, col, &oc)) fr0[fcount[0]++] = f[q];
-                    else adj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
+                    if (claim_mark(mark, f[q], tag, col, &oc)) {
+                        fr0[fcount[0]++] = f[q];
+                        pres[0] |= 1u << col;
+                    } else adj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
                 }
                 nbranch = nb;
                 if (nb <= 1) result = 1;
@@ -1406,18 +1440,15 @@ k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
                     for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
                     if (all) result = 1;
                     else {
-                        int pres = 0;
-                        const int* frc = cur == 0 ? fr0 : fr1;
-                        for (int e = 0; e < fcount[cur]; ++e)
-                            pres |= 1 << (__hip_atomic_load(mark + frc[e], __ATOMIC_RELAXED,
-                                                            __HIP_MEMORY_SCOPE_AGENT) & 7);
+                        const uint32_t present = pres[cur];
                         for (int q = 0; q < nb && result < 0; ++q) {
                             bool has = false;
-                            for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((pres >> p) & 1)) has = true;
+                            for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1u)) has = true;
                             if (!has) result = 0;
                         }
                     }
                     fcount[cur ^ 1] = 0;
+                    pres[cur ^ 1] = 0u;
                 }
                 __syncthreads();
                 if (result != -1) break;
@@ -1439,6 +1470,7 @@ k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
                         int oc = 0;
                         if (claim_mark(mark, nbc, tg, col, &oc)) {
                             frn[atomicAdd(&fcount[nxt], 1)] = nbc;
+                            atomicOr(&pres[nxt], 1u << col);
                         } else if (oc != col) {
                             atomicOr(&adj, (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc)));
                         }
@@ -1455,6 +1487,45 @@ k_flip_bfs_big(FlipCtx c, const int* __restrict__ fbq, int* __restrict__ mark,
         __syncthreads();
     }
     if (threadIdx.x == 0) c.s->fbcount = 0;
+}
+
+// One workgroup per queued candidate; candidates whose search outgrows the
+// LDS visited set go to the fallback queue, which the last workgroup to
+// finish runs with the device-memory search before the kernel ends.
+__global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, int round, int* __restrict__ fbq,
+                                                   int* __restrict__ mark, int* __restrict__ fr0,
+                                                   int* __restrict__ fr1, int* __restrict__ tag_ctr) {
+    __shared__ BfsShared S;
+    __shared__ int last;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c.s->qcount[(round + 1) & 1] = 0;
+    const int n = c.s->qcount[round & 1];
+    for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
+        const int k = c.queue[round & 1][qi];
+        const int flags = c.qflags[k];
+        const int cell = c.Ccell[k];
+        int ok = 1;
+        if (flags & 1) ok = bfs_connected_lds(S, c, cell, c.Cold[k], true, k);
+        if (ok == 1 && (flags & 2)) ok = bfs_connected_lds(S, c, cell, c.Cnew[k], false, k);
+        if (threadIdx.x == 0) {
+            if (ok < 0) {
+                const int p = atomicAdd(&c.s->fbcount, 1);
+                st_rel(fbq + p, k);
+            } else {
+                decide(c, k, ok ? ACCEPT : REJECT);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(&c.s->bfs_done, 1) == int(gridDim.x) - 1;
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        flip_bfs_fallback(c, fbq, mark, fr0, fr1, tag_ctr);
+        if (threadIdx.x == 0) st_rel(&c.s->bfs_done, 0);
+    }
 }
 
 // apply the decided transactions (src/gpk_transitions.cpp:300-312)

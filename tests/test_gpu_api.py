@@ -68,6 +68,29 @@ def test_topology_simple_move_matches_oracle():
             assert o.simple_move(int(cell), nl) == G.topology_simple_move(int(cell), nl), (cell, nl)
 
 
+def test_topology_simple_move_large_components():
+    """Connectivity searches that outgrow the LDS visited set (more than
+    6144 cells) and finish in the device-memory search.  Label 0 fills full
+    periodic rows; the cell c = (40, 50) of the first row is label 1.
+    One row: label 1 minus c is one cylinder (the top and bottom branches of
+    c meet around the torus), so c may take label 0.  Two rows: removing c
+    splits label 1 into two bands, so it may not."""
+    m = _abi.isotropic(1.0, 1.0)
+    for rows, expect in (((40,), True), ((40, 80), False)):
+        phi = np.ones((128, 128), np.int32)
+        for r in rows:
+            phi[r, :] = 0
+        phi[40, 50] = 1
+        o = O.OracleGrid(1, 128, 128, 1.0, m, 0.0, 1.5, "strict32")
+        G = gpk_amd.GPKGrid(1, 128, 128, 1.0, m, 0.0, 1.5)
+        o.set("phi", phi)
+        G.set("phi", phi)
+        cell = 40 * 128 + 50
+        assert o.simple_move(cell, 0) == expect
+        assert G.topology_simple_move(cell, 0) == expect, rows
+        G.close()
+
+
 # every case has cells outside their label's largest face component, so the
 # device connectivity normalisation re-attaches cells in each of them
 # (test_capi.test_voronoi_cases_have_detached_cells)
