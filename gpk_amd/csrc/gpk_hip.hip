@@ -993,11 +993,9 @@ struct Engine final : EngineBase {
         if (nb > 0) m = flips(nb);
         // without the topology test every candidate is accepted
         // (src/gpk_transitions.cpp:299-303): the count is known here
+        // (with it, the transaction's last read of the scalars holds the count)
         int accepted = m;
-        if (model.flags & GPK_FLAG_GRAIN_NETWORK) {
-            pull_scalars();
-            accepted = hs->accepted;
-        }
+        if ((model.flags & GPK_FLAG_GRAIN_NETWORK) && m > 0) accepted = hs->accepted;
         int exchanges = 0;
         if (accepted > 0) {
             exchanges = remap_sweep();
@@ -1210,9 +1208,8 @@ struct Engine final : EngineBase {
             }
         }
         if (!mask_fresh) update_mask();
-        if (npairs == 0) return 0;
-        pull_scalars();
-        return hs->exchanges;
+        // the pair rounds' last read of the scalars holds the exchange count
+        return npairs == 0 ? 0 : hs->exchanges;
     }
 
     // ---- unresolved-grain extinction (src/gpk_transitions.cpp:180-249) ----

@@ -1120,8 +1120,11 @@ struct FlipCtx {
     Scalars* s;
 };
 
+// a transaction's verdict, counted when it is taken so that the host's
+// last look at the undecided count also reads the accepted / refused totals
 __device__ __forceinline__ void decide(FlipCtx& c, int k, int v) {
     st_rel(c.state + k, v);
+    atomicAdd(v == ACCEPT ? &c.s->accepted : &c.s->deferred, 1);
     atomicSub(&c.s->undecided, 1);
 }
 
@@ -1555,9 +1558,10 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
             def = true;
         }
     }
-    // one counter update per wavefront
+    // one counter update per wavefront (transactions count their verdicts
+    // when they take them)
     const unsigned long long ba = __ballot(acc), bd = __ballot(def);
-    if ((threadIdx.x & 63) == 0) {
+    if (all_accept && (threadIdx.x & 63) == 0) {
         if (ba) atomicAdd(&s->accepted, __popcll(ba));
         if (bd) atomicAdd(&s->deferred, __popcll(bd));
     }
@@ -1878,6 +1882,7 @@ __global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __rest
         }
     }
     __hip_atomic_store(c.pstate + q, verdict, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (verdict == ACCEPT) atomicAdd(&c.s->exchanges, 2);
     atomicSub(&c.s->undecided, 1);
 }
 
@@ -1888,6 +1893,7 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
                              int* __restrict__ pid, Scalars* s, R* __restrict__ rho, R rho_rex,
                              int sweep) {
+    (void)s;   // exchanges are counted by the rounds that accept them
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= np) return;
     const int a = Pa[q], b = Pb[q];
@@ -1897,7 +1903,6 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
     g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
     alpha[a] = R(1); alpha[b] = R(1);
     if (sweep) { rho[a] = rho_rex; rho[b] = rho_rex; }   // src/gpk_transitions.cpp:173
-    atomicAdd(&s->exchanges, 2);
 }
 
 // remap-score cache: the (2r+1)^d box around every relabelled cell is marked
