@@ -430,7 +430,13 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         const int nbat = nr - b0 < S::MAXRES ? nr - b0 : S::MAXRES;
         const int bb = b0 / S::MAXRES;
         __syncthreads();
-        // row masks: wave per staged row, one ballot per resident of the batch
+        // row masks: wave per staged row, one ballot per resident of the
+        // batch; the INFL x MAXRES ballots of a pass are gathered into lanes
+        // k * MAXRES + t (branch-free selects) and stored by one LDS write
+        static_assert(INFL * S::MAXRES <= 64, "one lane per ballot of a pass");
+        int rv[S::MAXRES];
+#pragma unroll
+        for (int t = 0; t < S::MAXRES; ++t) rv[t] = t < nbat ? resl[b0 + t] : -2;   // -2: no label
         for (int r0 = wave; r0 < S::NROWS; r0 += INFL * nwave) {
             int v[INFL];
 #pragma unroll
@@ -439,16 +445,20 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                 v[k] = -1;
                 if (r < S::NROWS && lane < S::SX) v[k] = phi[zoff[r / S::SY] + yoff[r % S::SY] + xw];
             }
+            int mlo = 0, mhi = 0;
 #pragma unroll
-            for (int k = 0; k < INFL; ++k) {
-                const int r = r0 + k * nwave;
-                if (r < S::NROWS) {
-                    for (int t = 0; t < nbat; ++t) {
-                        const unsigned long long msk = __ballot(v[k] == resl[b0 + t]);
-                        if (lane == 0) rm[r * S::MAXRES + t] = msk;
-                    }
+            for (int k = 0; k < INFL; ++k)
+#pragma unroll
+                for (int t = 0; t < S::MAXRES; ++t) {
+                    const unsigned long long msk = __ballot(v[k] == rv[t]);
+                    const bool mine = lane == k * S::MAXRES + t;
+                    mlo = mine ? int(uint32_t(msk)) : mlo;
+                    mhi = mine ? int(uint32_t(msk >> 32)) : mhi;
                 }
-            }
+            const int kr = lane / S::MAXRES, tr = lane - kr * S::MAXRES;
+            const int r = r0 + kr * nwave;
+            if (lane < INFL * S::MAXRES && r < S::NROWS)
+                rm[r * S::MAXRES + tr] = (uint64_t(uint32_t(mhi)) << 32) | uint32_t(mlo);
         }
         __syncthreads();
         const int bend = boff[bb] + bcount[bb];
