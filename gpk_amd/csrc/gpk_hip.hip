@@ -1155,13 +1155,16 @@ struct Engine final : EngineBase {
                                                       kgain2.as<unsigned long long>(), idx.as<int>(),
                                                       order.as<int>(), np, 0, sizeof(R) == 4 ? 32 : 64,
                                                       stream));
-            k_prop_group_keys<<<nblk(np, 256), 256, 0, stream>>>(np, order.as<int>(), pcell.as<int>(),
+            // group keys hold two labels of hb bits each and the direction
+            int hb = 1;
+            while ((1LL << hb) < nlab) ++hb;
+            k_prop_group_keys<<<nblk(np, 256), 256, 0, stream>>>(np, hb, order.as<int>(), pcell.as<int>(),
                                                                   phi.as<int32_t>(), phi_comp.as<int32_t>(),
                                                                   kgrp.as<unsigned long long>());
             tb = cub_bytes;
             HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp.p, tb, kgrp.as<unsigned long long>(),
                                                       kgrp2.as<unsigned long long>(), order.as<int>(),
-                                                      order2.as<int>(), np, 0, 64, stream));
+                                                      order2.as<int>(), np, 0, 2 * hb + 1, stream));
             k_seg_bounds<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
                                                              seg_start.as<int>(), seg_end.as<int>());
             k_pair_flags<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
@@ -1177,7 +1180,7 @@ struct Engine final : EngineBase {
             npairs = tail[0] + tail[1];
             if (npairs > 0) {
                 k_pair_emit<R><<<nblk(np, 256), 256, 0, stream>>>(
-                    np, kgrp2.as<unsigned long long>(), seg_start.as<int>(), seg_end.as<int>(),
+                    np, hb, kgrp2.as<unsigned long long>(), seg_start.as<int>(), seg_end.as<int>(),
                     pflag2.as<int>(), pidx.as<int>(), order2.as<int>(), pcell.as<int>(), Pa.as<int>(),
                     Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(), pid.as<int>());
                 set_int(&ds()->undecided, npairs);

@@ -1621,7 +1621,8 @@ __global__ void k_prop_keys(int np, const int* __restrict__ sel, const int* __re
     kgain[k] = gb;
 }
 
-__global__ void k_prop_group_keys(int np, const int* __restrict__ order,
+// group key ((low << hb) | high) << 1 | dir, labels below 2^hb
+__global__ void k_prop_group_keys(int np, int hb, const int* __restrict__ order,
                                   const int* __restrict__ pcell, const int32_t* __restrict__ phi,
                                   const int32_t* __restrict__ phi_comp,
                                   unsigned long long* __restrict__ kgrp) {
@@ -1631,7 +1632,7 @@ __global__ void k_prop_group_keys(int np, const int* __restrict__ order,
     const int32_t p = phi[cell], q = phi_comp[cell];
     const uint32_t lo = uint32_t(p < q ? p : q), hi = uint32_t(p < q ? q : p);
     const uint32_t dir = p == int32_t(lo) ? 0u : 1u;
-    kgrp[s] = ((unsigned long long)lo << 27 | hi) << 1 | dir;
+    kgrp[s] = ((unsigned long long)lo << hb | hi) << 1 | dir;
 }
 
 // segments of equal (low, high, dir): start index per position
@@ -1669,7 +1670,7 @@ __global__ void k_pair_flags(int np, const unsigned long long* __restrict__ kg,
 }
 
 template <class R>
-__global__ void k_pair_emit(int np, const unsigned long long* __restrict__ kg,
+__global__ void k_pair_emit(int np, int hb, const unsigned long long* __restrict__ kg,
                             const int* __restrict__ seg_start, const int* __restrict__ seg_end,
                             const int* __restrict__ pflag, const int* __restrict__ pidx,
                             const int* __restrict__ order, const int* __restrict__ pcell,
@@ -1684,8 +1685,8 @@ __global__ void k_pair_emit(int np, const unsigned long long* __restrict__ kg,
     const int a = pcell[order[s]], b = pcell[order[rs + rnk]];
     const unsigned long long key = kg[s] >> 1;
     Pa[q] = a; Pb[q] = b;
-    Plo[q] = int(uint32_t(key >> 27));
-    Phi[q] = int(uint32_t(key & ((1ull << 27) - 1)));
+    Plo[q] = int(uint32_t(key >> hb));
+    Phi[q] = int(uint32_t(key & ((1ull << hb) - 1)));
     pstate[q] = UNDEC;
     pid[a] = q; pid[b] = q;
 }
