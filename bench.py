@@ -22,6 +22,12 @@ Omega 1e8) at c = c_sat (pure capillary flow), D = 0, sigma 1.5; 300 steps
 in FP32 and FP64 through the batched persistent kernel after the autotuner
 has picked its launch shape.  value = instance-steps/s.
 
+`--workload batch_grain` runs 512 2-D grain-growth instances (side
+~ U_int[64, 256], numpy default_rng(7); one periodic Voronoi grain per 64
+cells from the library's host generator), isotropic growth with the
+grain-network topology test, sigma 1.5, 200 steps in FP32 and FP64 through
+the same batched kernel.  value = instance-steps/s.
+
 `--workload wide_sigma` runs 128x256x256 grain growth at sigma 10.67 cells
 (240 grains), the per-label separable geometry path.
 
@@ -215,11 +221,16 @@ def batch_model():
     return _abi.aniso_ostwald(1.0, 1.0, 1.0, 0.5, 1.0, float(np.float32(0.1)), 1e8)
 
 
-def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20):
+def lamellar_fields(shapes):
+    return {"c": [np.full(s, np.float32(0.1), np.float32) for s in shapes]}
+
+
+def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20, model=None, fields=None):
     from gpk_amd.batch import GPKBatch
-    B = GPKBatch(shapes, 1.0, batch_model(), 0.0, sigma, precision=prec)
+    B = GPKBatch(shapes, 1.0, model or batch_model(), 0.0, sigma, precision=prec)
     B.set("phi", phis)
-    B.set("c", [np.full(s, np.float32(0.1), np.float32) for s in shapes])
+    for name, arrays in (fields if fields is not None else lamellar_fields(shapes)).items():
+        B.set(name, arrays)
     t0 = time.perf_counter()
     wg, nwg, tune_ms = B.autotune()
     t_tune = time.perf_counter() - t0
@@ -241,6 +252,7 @@ def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20):
         "ms_per_step": 1e3 * wall / steps, "instance_steps_per_s": n * steps / wall,
         "cell_updates_per_s": st["band_cells"] * steps / (steps + 1) / wall,
         "accepted": int(st["accepted"]), "exchanges": int(st["exchanges"]),
+        "deferred": int(st["deferred"]), "eroded": int(st["eroded"]),
         "guard_band_decisions": int(st["guarded"]),
         "instance_step_ms_p50": float(np.percentile(per, 50)),
         "instance_step_ms_p99": float(np.percentile(per, 99)),
@@ -248,18 +260,21 @@ def run_batch(prec, shapes, phis, steps, sigma, sample_steps=20):
     }
 
 
-def cpu_batch_baseline(shapes, phis, sigma, n_inst=32, steps=30):
+def cpu_batch_baseline(shapes, phis, sigma, n_inst=32, steps=30, model=None, fields=None):
     """Reference GPKGrid steps (compiled reference sources) on a sample of instances."""
     from oracle import oracle as O
     kind = "reference" if O.ref_available() else "port"
+    model = model or batch_model()
+    fields = fields if fields is not None else lamellar_fields(shapes)
     wall = 0.0
-    for s, p in list(zip(shapes, phis))[:n_inst]:
+    for b, (s, p) in enumerate(list(zip(shapes, phis))[:n_inst]):
         if kind == "reference":
-            R = O.RefGrid(1, s[0], s[1], 1.0, batch_model(), 0.0, sigma)
+            R = O.RefGrid(1, s[0], s[1], 1.0, model, 0.0, sigma)
         else:
-            R = O.OracleGrid(1, s[0], s[1], 1.0, batch_model(), 0.0, sigma, "strict32")
+            R = O.OracleGrid(1, s[0], s[1], 1.0, model, 0.0, sigma, "strict32")
         R.set("phi", p)
-        R.set("c", np.full(s, np.float32(0.1), np.float32))
+        for name, arrays in fields.items():
+            R.set(name, arrays[b])
         t0 = time.perf_counter()
         for _ in range(steps):
             R.step()
@@ -288,6 +303,48 @@ def main_batch(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded lamellar sweep)",
         "config": {"workload": "512 2-D instances, side U[64,384], lamellae t U[6,20], period r t, "
                                "r U[2,12], AnisoOstwald c = c_sat, %d steps" % steps,
+                   "instances": n, "cells": int(sum(a.size for a in p32)), "precision": "fp32",
+                   "parallelism": "single GPU"},
+        "cpu_baseline": cpu, "fp32": r32, "fp64": r64,
+    }
+    if cpu:
+        out["speedup_vs_cpu"] = r32["instance_steps_per_s"] / cpu["value"]
+    print(json.dumps(out))
+
+
+def main_batch_grain(args):
+    """Batched 2-D isotropic grain growth with the grain-network topology
+    test: 512 instances of side U[64, 256] (seeded), each a periodic Voronoi
+    map of one grain per 64 cells on average (host Voronoi generator of the
+    library), orientations from the generator; flips decided one after
+    another per instance, unresolved grains eroded every 8th step."""
+    import gpk_amd
+    from gpk_amd import _abi
+    n = 512
+    steps = args.steps if args.steps != 20 else 200
+    rng = np.random.default_rng(7)
+    shapes, phis, gs = [], [], []
+    for b in range(n):
+        ny, nx = (int(v) for v in rng.integers(64, 257, 2))
+        ng = max(4, min(1000, ny * nx // 64))
+        p, g = gpk_amd.voronoi_host(1, ny, nx, ng, 1000 + b)
+        shapes.append((ny, nx))
+        phis.append(p.reshape(ny, nx))
+        gs.append(g.reshape(ny, nx))
+    model = _abi.isotropic(1.0, 1.0)
+    fields = {"g": gs}
+    p32, r32 = run_batch("fp32", shapes, phis, steps, args.sigma, model=model, fields=fields)
+    p64, r64 = run_batch("fp64", shapes, phis, steps, args.sigma, model=model, fields=fields)
+    diff = sum(int(np.count_nonzero(a != b)) for a, b in zip(p32, p64))
+    r32["label_disagreement_vs_fp64"] = diff / float(sum(a.size for a in p32))
+    cpu = None if args.no_cpu else cpu_batch_baseline(shapes, phis, args.sigma, model=model, fields=fields)
+    out = {
+        "metric": "instance-steps/s, 512 batched 2-D grain-growth instances (grain-network topology test)",
+        "value": r32["instance_steps_per_s"], "unit": "instance-steps/s", "n_gpus": 1, "steps": steps,
+        "warmup": 1, "ms_per_step": r32["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded periodic Voronoi maps)",
+        "config": {"workload": "512 2-D instances, side U[64,256], one Voronoi grain per 64 cells, "
+                               "isotropic growth, sigma %.2f, %d steps" % (args.sigma, steps),
                    "instances": n, "cells": int(sum(a.size for a in p32)), "precision": "fp32",
                    "parallelism": "single GPU"},
         "cpu_baseline": cpu, "fp32": r32, "fp64": r64,
@@ -337,7 +394,8 @@ def main_wide_sigma(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["grain_growth", "solidification", "batch", "wide_sigma"],
+    ap.add_argument("--workload", choices=["grain_growth", "solidification", "batch", "batch_grain",
+                                           "wide_sigma"],
                     default="grain_growth")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -358,6 +416,9 @@ def main():
         return
     if args.workload == "batch":
         main_batch(args)
+        return
+    if args.workload == "batch_grain":
+        main_batch_grain(args)
         return
     if args.workload == "wide_sigma":
         main_wide_sigma(args)

@@ -396,8 +396,12 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
             }
         }
     } else {
-        // candidates in band (= index) order; alpha_new stored for all
+        // candidates in band (= index) order; alpha_new stored for all.  The
+        // band list is not read again in this step before the remap rebuilds
+        // it, so its front takes each candidate's old label (read here, before
+        // any flip, since a candidate is the only writer of its own label)
         int* cand = reinterpret_cast<int*>(props);   // sort scratch: 4 ints per cell
+        int32_t* cand_old = band;
         int* mark = cand + I.spow;
         int* fr0 = cand + 2 * I.spow;
         int* fr1 = cand + 3 * I.spow;
@@ -416,9 +420,13 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
                 alpha[i] = an;
                 f = below && phi_comp[i] != phi[i];
             }
+            const int32_t pi = f ? phi[i] : 0;
             int tot;
-            const int pos = block_scan_flag<WG>(sh, f, tot);
-            if (f) cand[nc + pos] = i;
+            const int pos = block_scan_flag<WG>(sh, f, tot);   // every band[a] of the pass read before
+            if (f) {
+                cand[nc + pos] = i;
+                cand_old[nc + pos] = pi;
+            }
             nc += tot;
         }
         // label counts, kept current as flips are accepted
@@ -427,26 +435,33 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
         for (int i = tid; i < d.N; i += WG) atomicAdd(&sh.hist[phi[i]], 1);
         __syncthreads();
         // one candidate after another, each on the labels the earlier ones
-        // left; every thread evaluates the same decision so that the
-        // connectivity searches are entered by the whole workgroup
+        // left.  Every thread evaluates the 3x3 numbers (the same values) so
+        // that the connectivity searches are entered by the whole workgroup;
+        // only thread 0 reads the label counts, which it alone updates, so
+        // one barrier per candidate orders the updates.  (With oo == 0 the
+        // count rule may refuse a candidate whose new-label search then ran
+        // for nothing; the verdict is the same.)
         int tag = 0;
         for (int e = 0; e < nc; ++e) {
+            // ol from the list: thread 0 may already have relabelled cell i
+            // while a slower wavefront starts this iteration
             const int i = cand[e];
-            const int32_t ol = phi[i], nl = phi_comp[i];
+            const int32_t ol = cand_old[e], nl = phi_comp[i];
             uint32_t co, cn;
             cube_masks(d, phi, i, ol, nl, co, cn);
             const int oo = topo_count(co, false), no = topo_count(cn, false);
-            // src/gpk_connectivity.cpp:179-206: a vanishing old label only
-            // when it is the label's last cell, a new label that touches
-            bool ok = no > 0 && !(oo == 0 && sh.hist[ol] > 1);
+            // src/gpk_connectivity.cpp:179-206: a new label that touches the
+            // cell, connected branches of both labels, and (below) a
+            // vanishing old label only when it is the label's last cell
+            bool ok = no > 0;
             if (ok && (oo > 1 || no > 1) && tag == 0) {
                 for (int c2 = tid; c2 < d.N; c2 += WG) mark[c2] = 0;
                 __syncthreads();
             }
             if (ok && oo > 1) ok = batch_connected<WG>(sh, d, phi, i, ol, true, mark, fr0, fr1, ++tag);
             if (ok && no > 1) ok = batch_connected<WG>(sh, d, phi, i, nl, false, mark, fr0, fr1, ++tag);
-            __syncthreads();
             if (tid == 0) {
+                ok = ok && !(oo == 0 && sh.hist[ol] > 1);
                 if (ok) {
                     phi[i] = nl;
                     g[i] = g_comp[i];
@@ -458,8 +473,10 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
                     alpha[i] = an < Eps<R>::flip ? Eps<R>::flip : an;
                 }
             }
-            acc += (ok && tid == 0) ? 1 : 0;
-            deferred += (!ok && tid == 0) ? 1 : 0;
+            if (tid == 0) {
+                acc += ok ? 1 : 0;
+                deferred += ok ? 0 : 1;
+            }
             __syncthreads();
         }
     }
