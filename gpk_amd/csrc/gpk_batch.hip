@@ -234,6 +234,12 @@ __device__ bool batch_connected(Shared<WG>& sh, const Dims& d, const int32_t* __
     __syncthreads();
     int cur = 0;
     while (true) {
+        // thread 0 judges the level just finished and clears the counters of
+        // the next frontier (last read two levels ago) in the same turn
+        if (tid == 0) {
+            sh.bfs_cnt[cur ^ 1] = 0;
+            sh.bfs_present[cur ^ 1] = 0u;
+        }
         if (tid == 0 && sh.bfs_res < 0) {
             const int nb = sh.bfs_nb;
             const uint32_t adj = sh.bfs_adj;
@@ -269,11 +275,6 @@ __device__ bool batch_connected(Shared<WG>& sh, const Dims& d, const int32_t* __
         const int* fin = cur ? fr1 : fr0;
         int* fout = cur ? fr0 : fr1;
         const int n = sh.bfs_cnt[cur];
-        if (tid == 0) {
-            sh.bfs_cnt[nxt] = 0;
-            sh.bfs_present[nxt] = 0u;
-        }
-        __syncthreads();
         for (int e = tid; e < n; e += WG) {
             const int v = fin[e];
             const int col = v & 7;
