@@ -587,10 +587,11 @@ __device__ __forceinline__ int xcd_tile(int ntiles) {
     return (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
 }
 
-// +0.0 unless the predicate holds (bit mask, no branch)
-__device__ __forceinline__ double keep_if(bool hit, double v) {
-    const unsigned long long m = hit ? ~0ull : 0ull;
-    return __longlong_as_double((long long)((unsigned long long)__double_as_longlong(v) & m));
+// acc + v when hit, acc otherwise, as one select and one FMA: fma(v, 1, acc)
+// rounds acc + v exactly as the add does, and fma(v, 0, acc) adds +-0, which
+// leaves acc unchanged (the window sums start at +0 and never hold -0)
+__device__ __forceinline__ double add_if(bool hit, double v, double acc) {
+    return __builtin_fma(v, hit ? 1.0 : 0.0, acc);
 }
 
 // same-pair projection of v (src/gpk_physics.cpp:152-193) from LDS
@@ -691,7 +692,7 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
                     const int dy = jy - RP;
                     const int rowb = ((lz + H::HZ + dz) * H::SY + (ly + RP + dy)) * H::SX + lx;
                     // the row's labels and velocities first, then the sums in
-                    // window order; a miss adds +0.0, which leaves the sums
+                    // window order; a miss adds +-0, which leaves the sums
                     // unchanged (they never hold -0)
                     uint32_t kj[W];
                     R vj[W];
@@ -708,8 +709,8 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
                         const double t = __longlong_as_double(
                             __double_as_longlong(w * double(vj[jx])) ^
                             (static_cast<long long>(kj[jx] & 1u) << 63));
-                        num += keep_if(hit, t);
-                        den += keep_if(hit, w);
+                        num = add_if(hit, t, num);
+                        den = add_if(hit, w, den);
                     }
                 }
             }
@@ -784,11 +785,11 @@ k_project_keyed(Dims d, int r, int nband, const int* __restrict__ band, const ui
                         const uint32_t kj = krow[x];
                         const bool hit = (kj >> 1) == pair;
                         const double w = wr[ddx * ddx];
-                        // (-w) v == -(w v) exactly; a miss adds +0.0
+                        // (-w) v == -(w v) exactly; a miss adds +-0
                         const double t = __longlong_as_double(__double_as_longlong(w * double(vrow[x])) ^
                                                               (static_cast<long long>(kj & 1u) << 63));
-                        num += keep_if(hit, t);
-                        den += keep_if(hit, w);
+                        num = add_if(hit, t, num);
+                        den = add_if(hit, w, den);
                         if (++x == d.Nx) x = 0;
                     }
                 }
@@ -861,9 +862,9 @@ __device__ __forceinline__ void remap_window(const uint16_t* lab, const double* 
             for (int jx = 0; jx < W; ++jx) {
                 const int dxx = jx - RS;
                 const double w = wr[dxx * dxx];
-                acc[0] += keep_if(v[jx] == p, w);
+                acc[0] = add_if(v[jx] == p, w, acc[0]);
 #pragma unroll
-                for (int t = 0; t < NC; ++t) acc[1 + t] += keep_if(v[jx] == L[1 + t], w);
+                for (int t = 0; t < NC; ++t) acc[1 + t] = add_if(v[jx] == L[1 + t], w, acc[1 + t]);
             }
         }
     }
