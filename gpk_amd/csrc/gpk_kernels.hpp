@@ -657,12 +657,15 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
             const int e = e0 + k * blockDim.x;
             if (e < H::NS) {
                 uint32_t kk = 0xFFFFFFFFu;
+                R vs = v[k];
                 if (mk[k] && p[k] != q[k]) {
                     const uint32_t lo = p[k] < q[k] ? p[k] : q[k], hi = p[k] < q[k] ? q[k] : p[k];
                     kk = (((lo << 16) | hi) << 1) | (p[k] == lo ? 0u : 1u);
+                    // staged with its orientation: w (-v) == -(w v) exactly
+                    vs = p[k] == lo ? vs : -vs;
                 }
                 key[e] = kk;
-                vv[e] = v[k];
+                vv[e] = vs;
             }
         }
     }
@@ -704,11 +707,7 @@ k_project_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ m
                         const int dxx = jx - RP;
                         const bool hit = (kj[jx] >> 1) == pair;
                         const double w = wr[dxx * dxx];
-                        // (-w) v == -(w v) exactly: the orientation flips the
-                        // product's sign bit
-                        const double t = __longlong_as_double(
-                            __double_as_longlong(w * double(vj[jx])) ^
-                            (static_cast<long long>(kj[jx] & 1u) << 63));
+                        const double t = w * double(vj[jx]);
                         num = add_if(hit, t, num);
                         den = add_if(hit, w, den);
                     }
