@@ -234,7 +234,7 @@ __device__ bool batch_connected(Shared<WG>& sh, const Dims& d, const int32_t* __
     __syncthreads();
     int cur = 0;
     while (true) {
-        // thread 0 judges the level just finished and clears the counters of
+        // thread 0 checks the level just finished and clears the counters of
         // the next frontier (last read two levels ago) in the same turn
         if (tid == 0) {
             sh.bfs_cnt[cur ^ 1] = 0;
