@@ -342,21 +342,46 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     for (int e = threadIdx.x; e < S::SZ; e += blockDim.x) zoff[e] = wrapi(z0t + e - S::HZ, d.Nz) * d.Ny * d.Nx;
     for (int e = threadIdx.x; e < S::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RG, d.Ny) * d.Nx;
     __syncthreads();
-    // interface cells of the tile and their distinct resident labels
-    for (int e = threadIdx.x; e < NT; e += blockDim.x) {
-        const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
-        const int x = x0t + lx, y = y0t + ly, z = z0t + lz;
-        if (x < d.Nx && y < d.Ny && z < d.Nz) {
-            const int i = lin(d, z, y, x);
-            if (mask[i]) {
-                list[atomicAdd(&nlist, 1)] = short(e);
-                const int p = phi[i];
-                int h = int((uint32_t(p) * 2654435761u) >> 23) & (GEO_HCAP - 1);
-                for (int probe = 0; probe < GEO_HCAP; ++probe) {
-                    const int prev = atomicCAS(&hkey[h], -1, p);
-                    if (prev == -1 || prev == p) break;
-                    h = (h + 1) & (GEO_HCAP - 1);
+    // interface cells of the tile and their distinct resident labels, one
+    // thread per z column of the tile; in 3-D the column's band cells are
+    // paired greedily from the bottom: a band cell heads a pair with the band
+    // cell above it when both carry the same label and it is not itself the
+    // upper cell of a pair (zpair: 1 head, 2 taken; the batch list's memory,
+    // not used before the list is built)
+    uint8_t* const zpair = reinterpret_cast<uint8_t*>(blist);
+    static_assert(sizeof(blist) >= NT, "pair flags alias the batch list");
+    for (int col = threadIdx.x; col < S::TY * S::TX; col += blockDim.x) {
+        const int lx = col % S::TX, ly = col / S::TX;
+        const int x = x0t + lx, y = y0t + ly;
+        int plab = -1;          // label of the previous band cell if it is free to head a pair
+        for (int lz = 0; lz < S::TZ; ++lz) {
+            const int e = lz * (S::TY * S::TX) + col;
+            const int z = z0t + lz;
+            int lab = -1;
+            if (x < d.Nx && y < d.Ny && z < d.Nz) {
+                const int i = lin(d, z, y, x);
+                if (mask[i]) {
+                    list[atomicAdd(&nlist, 1)] = short(e);
+                    const int p = phi[i];
+                    lab = p;
+                    int h = int((uint32_t(p) * 2654435761u) >> 23) & (GEO_HCAP - 1);
+                    for (int probe = 0; probe < GEO_HCAP; ++probe) {
+                        const int prev = atomicCAS(&hkey[h], -1, p);
+                        if (prev == -1 || prev == p) break;
+                        h = (h + 1) & (GEO_HCAP - 1);
+                    }
                 }
+            }
+            if (DIM == 3) {
+                uint8_t f = 0;
+                if (lab >= 0 && plab == lab) {
+                    f = 2;                       // upper cell of the pair headed at lz - 1
+                    zpair[e - S::TY * S::TX] = 1;
+                    plab = -1;
+                } else {
+                    plab = lab;
+                }
+                zpair[e] = f;
             }
         }
     }
@@ -381,20 +406,12 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
             if (hkey[h] == -1) break;
             h = (h + 1) & (GEO_HCAP - 1);
         }
-        // 3-D: a cell at even lz takes its band neighbour at lz + 1 when both
-        // carry the same label (bit 8: head of a pair, bit 9: taken by one)
+        // 3-D pairs from the column pass (bit 8: head of a pair, bit 9:
+        // taken by one)
         unsigned short fl = 0;
         if (DIM == 3 && r != 0xFFFFu) {
-            const int zg = z0t + lz;
-            if ((lz & 1) == 0) {
-                if (lz + 1 < S::TZ && zg + 1 < d.Nz) {
-                    const int ip = lin(d, zg + 1, y0t + ly, x0t + lx);
-                    if (mask[ip] && phi[ip] == p) fl = 0x100;
-                }
-            } else {
-                const int ip = lin(d, zg - 1, y0t + ly, x0t + lx);
-                if (mask[ip] && phi[ip] == p) fl = 0x200;
-            }
+            const uint8_t zf = zpair[e];
+            fl = zf == 1 ? 0x100 : (zf == 2 ? 0x200 : 0);
         }
         cres[q] = r == 0xFFFFu ? r : (unsigned short)(r | fl);
         if (!(fl & 0x200)) atomicAdd(&bcount[r == 0xFFFFu ? S::MAXB : r / S::MAXRES], 1);
