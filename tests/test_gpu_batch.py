@@ -142,3 +142,37 @@ def test_batch_rejects_unsupported_models():
         GPKBatch([(32, 32)], 1.0, _abi.dislocation(1.0, 1.0, 1.0, 1.0), 0.0, 1.5, precision="fp32")
     with pytest.raises(Exception):
         GPKBatch([(32, 32)], 1.0, model(), 1e-9, 1.5, precision="fp32")
+
+
+def test_batch_grain_network_larger_instances():
+    """Grain growth on larger instances (up to 128^2, longer connectivity
+    searches through bigger grains), FP64, against the oracle bit for bit."""
+    cases = [(128, 128, 220), (96, 160, 200), (120, 72, 150)]
+    shapes = [(ny, nx) for ny, nx, _ in cases]
+    phis, gs = [], []
+    for k, (ny, nx, ns) in enumerate(cases):
+        p, g = O.voronoi(1, ny, nx, ns, 77 + k)
+        phis.append(p.reshape(ny, nx))
+        gs.append(g.reshape(ny, nx))
+    steps = 24
+    thermo = _abi.isotropic(1.0, 1.0)
+    B = GPKBatch(shapes, 1.0, thermo, 0.0, 1.5, precision="fp64")
+    B.set("phi", phis)
+    B.set("g", gs)
+    B.set("alpha", [np.ones(s, np.float32) for s in shapes])
+    print("batch grain network, larger instances", cases, flush=True)
+    B.step(steps)
+    got = {f: B.get(f) for f in ("phi", "alpha", "g")}
+    st = B.stats()
+    deferred = 0
+    for b, (shape, phi) in enumerate(zip(shapes, phis)):
+        o = O.OracleGrid(1, shape[0], shape[1], 1.0, thermo, 0.0, 1.5, "fp64")
+        o.set("phi", phi)
+        o.set("g", gs[b].astype(o.real))
+        for _ in range(steps):
+            o.step()
+            deferred += o.stats()["deferred"]
+        for f in ("phi", "alpha", "g"):
+            assert np.array_equal(got[f][b], o.get(f)), (b, f)
+    assert deferred > 0 and st["deferred"] == deferred
+    B.close()
