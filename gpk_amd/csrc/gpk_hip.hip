@@ -1442,13 +1442,43 @@ struct Engine final : EngineBase {
         HIPCHK(hipMemcpyAsync(sz.p, S.z.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(sy.p, S.y.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(sx.p, S.x.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
-        const bool narrow = std::max({Nz, Ny, Nx}) <= 65536;
-        if (narrow)
-            k_voronoi<uint32_t><<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(dims, n_seeds, sz.as<int>(), sy.as<int>(),
-                                                                              sx.as<int>(), phi.as<int32_t>());
-        else
-            k_voronoi<unsigned long long><<<nblk(N, 256), 256, 3 * 1024 * 4, stream>>>(
-                dims, n_seeds, sz.as<int>(), sy.as<int>(), sx.as<int>(), phi.as<int32_t>());
+        // bucket grid of the sites (the same grid as the host search), sites of
+        // each bucket in ascending index order
+        SiteBuckets sb{};
+        {
+            const double per = std::max(1.0, double(N) / n_seeds);
+            const int B = std::max(1, Nz == 1 ? static_cast<int>(std::sqrt(per)) : static_cast<int>(std::cbrt(per)));
+            sb.Bz = Nz == 1 ? Nz : std::min(B, Nz);
+            sb.By = std::min(B, Ny);
+            sb.Bx = std::min(B, Nx);
+            sb.bz = (Nz + sb.Bz - 1) / sb.Bz;
+            sb.by = (Ny + sb.By - 1) / sb.By;
+            sb.bx = (Nx + sb.Bx - 1) / sb.Bx;
+            int bmin = std::min(Ny - (sb.by - 1) * sb.By, sb.By);
+            bmin = std::min(bmin, std::min(Nx - (sb.bx - 1) * sb.Bx, sb.Bx));
+            if (Nz > 1) bmin = std::min(bmin, std::min(Nz - (sb.bz - 1) * sb.Bz, sb.Bz));
+            sb.bmin = std::max(bmin, 1);
+        }
+        const size_t nbk = size_t(sb.bz) * sb.by * sb.bx;
+        std::vector<int> bstart(nbk + 1, 0);
+        std::vector<int> bsites(static_cast<size_t>(n_seeds));
+        auto bucket_of = [&](int k) {
+            return (size_t(S.z[k] / sb.Bz) * sb.by + S.y[k] / sb.By) * sb.bx + S.x[k] / sb.Bx;
+        };
+        for (int k = 0; k < n_seeds; ++k) ++bstart[bucket_of(k) + 1];
+        for (size_t q = 0; q < nbk; ++q) bstart[q + 1] += bstart[q];
+        {
+            std::vector<int> fillp(bstart.begin(), bstart.end() - 1);
+            for (int k = 0; k < n_seeds; ++k) bsites[size_t(fillp[bucket_of(k)]++)] = k;
+        }
+        DevBuf dbs, dbi;
+        dbs.alloc((nbk + 1) * 4);
+        dbi.alloc(size_t(n_seeds) * 4);
+        HIPCHK(hipMemcpyAsync(dbs.p, bstart.data(), (nbk + 1) * 4, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dbi.p, bsites.data(), size_t(n_seeds) * 4, hipMemcpyHostToDevice, stream));
+        k_voronoi_buckets<<<nblk(N, 256), 256, 0, stream>>>(dims, sb, sz.as<int>(), sy.as<int>(), sx.as<int>(),
+                                                           dbs.as<int>(), dbi.as<int>(), phi.as<int32_t>());
+        HIPCHK(hipGetLastError());
         normalise_device(n_seeds, S);
         fill<R>(alpha, N, R(1));
         check_labels();

@@ -2389,42 +2389,67 @@ __global__ void k_count_pos(int n, const int* __restrict__ cnt, int* __restrict_
 
 // ---------------------------------------------------------------------------
 // Voronoi nearest site (demos/gpk.cpp:72-89): exact integer periodic squared
-// distance, lowest site index on ties.
+// distance, lowest site index on ties (the reference's strict < scan in site
+// order, demos/gpk.cpp:51-120).
 // ---------------------------------------------------------------------------
-// nearest site (periodic squared distance, strict <: the lowest site index
-// wins ties, demos/gpk.cpp:51-120); U is the distance type, 32-bit when every
-// grid side is at most 65536 (3 * 32768^2 < 2^32)
-template <class U>
-__global__ void k_voronoi(Dims d, int ns, const int* __restrict__ sz, const int* __restrict__ sy,
-                          const int* __restrict__ sx, int32_t* __restrict__ phi) {
-    extern __shared__ int sh[];
-    int* lz = sh;
-    int* ly = sh + 1024;
-    int* lx = sh + 2048;
+
+// Nearest site from a bucket grid of the sites: a site in a bucket
+// at periodic bucket distance k >= ring + 1 along some axis lies at least
+// ring * bmin + 1 cells away along that axis, so the ring search stops once
+// the best squared distance is below that bound (a site at the bound is
+// still visited and the lowest index wins ties, as in the plain scan).
+// bstart / bsites: the sites of each bucket in ascending index order.
+struct SiteBuckets {
+    int Bz, By, Bx;       // bucket extent per axis
+    int bz, by, bx;       // buckets per axis
+    int bmin;             // smallest bucket extent over the searched axes
+};
+__global__ void k_voronoi_buckets(Dims d, SiteBuckets sb, const int* __restrict__ sz,
+                                  const int* __restrict__ sy, const int* __restrict__ sx,
+                                  const int* __restrict__ bstart, const int* __restrict__ bsites,
+                                  int32_t* __restrict__ phi) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int z = 0, y = 0, x = 0;
-    if (i < d.N) decode(d, i, z, y, x);
-    U bd = ~U(0);
+    if (i >= d.N) return;
+    int z, y, x;
+    decode(d, i, z, y, x);
+    const bool d3 = d.Nz > 1;
+    const int cz = z / sb.Bz, cy = y / sb.By, cx = x / sb.Bx;
+    long long bd = 0x7FFFFFFFFFFFFFFFLL;
     int best = 0;
-    for (int base = 0; base < ns; base += 1024) {
-        const int cnt = ns - base < 1024 ? ns - base : 1024;
-        __syncthreads();
-        for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
-            lz[t] = sz[base + t]; ly[t] = sy[base + t]; lx[t] = sx[base + t];
-        }
-        __syncthreads();
-        for (int t = 0; t < cnt; ++t) {
-            int a = z - lz[t]; a = a < 0 ? -a : a;
-            int b = y - ly[t]; b = b < 0 ? -b : b;
-            int e = x - lx[t]; e = e < 0 ? -e : e;
-            a = a < d.Nz - a ? a : d.Nz - a;
-            b = b < d.Ny - b ? b : d.Ny - b;
-            e = e < d.Nx - e ? e : d.Nx - e;
-            const U dd = U(a) * U(a) + U(b) * U(b) + U(e) * U(e);
-            if (dd < bd) { bd = dd; best = base + t; }
-        }
+    for (int ring = 0;; ++ring) {
+        const int rz = d3 ? ring : 0;
+        for (int dz = -rz; dz <= rz; ++dz)
+            for (int dy = -ring; dy <= ring; ++dy)
+                for (int dxb = -ring; dxb <= ring; ++dxb) {
+                    const int az = dz < 0 ? -dz : dz, ay = dy < 0 ? -dy : dy, ax = dxb < 0 ? -dxb : dxb;
+                    const int m = az > ay ? (az > ax ? az : ax) : (ay > ax ? ay : ax);
+                    if (m != ring) continue;
+                    const int qz = ((cz + dz) % sb.bz + sb.bz) % sb.bz;
+                    const int qy = ((cy + dy) % sb.by + sb.by) % sb.by;
+                    const int qx = ((cx + dxb) % sb.bx + sb.bx) % sb.bx;
+                    const int q = (qz * sb.by + qy) * sb.bx + qx;
+                    for (int t = bstart[q]; t < bstart[q + 1]; ++t) {
+                        const int k = bsites[t];
+                        long long a = z - sz[k], b = y - sy[k], e = x - sx[k];
+                        a = a < 0 ? -a : a;
+                        b = b < 0 ? -b : b;
+                        e = e < 0 ? -e : e;
+                        a = a < d.Nz - a ? a : d.Nz - a;
+                        b = b < d.Ny - b ? b : d.Ny - b;
+                        e = e < d.Nx - e ? e : d.Nx - e;
+                        const long long dd = a * a + b * b + e * e;
+                        if (dd < bd || (dd == bd && k < best)) {
+                            bd = dd;
+                            best = k;
+                        }
+                    }
+                }
+        const bool all = 2 * ring + 1 >= sb.bx && 2 * ring + 1 >= sb.by && (!d3 || 2 * ring + 1 >= sb.bz);
+        if (all) break;
+        const long long lb = (long long)ring * sb.bmin + 1;
+        if (bd < lb * lb) break;
     }
-    if (i < d.N) phi[i] = best + 1;
+    phi[i] = best + 1;
 }
 
 // ---------------------------------------------------------------------------
