@@ -44,7 +44,7 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Voronoi generator (demos/gpk.cpp:51-189).  Site draws use std::mt19937 and
-// the libstdc++ distributions; the nearest-site labels come from k_voronoi;
+// the libstdc++ distributions; the nearest-site labels come from k_voronoi_buckets;
 // connectivity normalisation runs on the host over the label array.
 // ---------------------------------------------------------------------------
 struct Sites {
