@@ -44,8 +44,9 @@ namespace {
 
 // ---------------------------------------------------------------------------
 // Voronoi generator (demos/gpk.cpp:51-189).  Site draws use std::mt19937 and
-// the libstdc++ distributions; the nearest-site labels come from k_voronoi_buckets;
-// connectivity normalisation runs on the host over the label array.
+// the libstdc++ distributions.  On a grid the nearest-site labels come from
+// k_voronoi_buckets and the connectivity normalisation runs on the device;
+// gpk_voronoi_host does both on the host (the functions below).
 // ---------------------------------------------------------------------------
 struct Sites {
     std::vector<int> z, y, x;
