@@ -1251,9 +1251,9 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
 // branch at once and stopping when all colours have met or one merged colour
 // group can grow no further.  One workgroup; LDS hash of visited cells.
 // ---------------------------------------------------------------------------
-constexpr int BFS_HCAP = 8192;      // hash slots (power of two)
-constexpr int BFS_HMAX = 6144;      // max insertions before overflow
-constexpr int BFS_FCAP = 2048;      // frontier capacity
+constexpr int BFS_HCAP = 4096;      // hash slots (power of two; 44 KB of LDS in all: three workgroups per CU)
+constexpr int BFS_HMAX = 3072;      // max insertions before overflow
+constexpr int BFS_FCAP = 1024;      // frontier capacity
 
 struct BfsShared {
     unsigned long long hash[BFS_HCAP];
@@ -1269,7 +1269,7 @@ struct BfsShared {
 };
 
 __device__ __forceinline__ uint32_t hslot(int cell) {
-    return (uint32_t(cell) * 2654435761u) >> (32 - 13);
+    return (uint32_t(cell) * 2654435761u) >> (32 - 12);
 }
 
 // returns 1 connected, 0 disconnected, -1 overflow
