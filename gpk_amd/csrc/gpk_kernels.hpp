@@ -887,7 +887,7 @@ __device__ __forceinline__ void remap_window(const uint16_t* lab, const double* 
 }
 
 template <class R, int DIM, int RS>
-__global__ void __launch_bounds__(256, 6)
+__global__ void __launch_bounds__(256, 8)
 k_remap_score_tiled(Dims d, int ntx, int nty, int ntz, const uint8_t* __restrict__ mask,
                     const int32_t* __restrict__ phi, const int* __restrict__ slot,
                     const double* __restrict__ wtab_g, int32_t* __restrict__ phi_comp,
