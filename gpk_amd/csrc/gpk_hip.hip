@@ -273,6 +273,18 @@ struct Engine final : EngineBase {
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt, pkey, pvfull;
     bool mask_fresh = false;    // mask matches the current labels
     int flip_batch = 4, pair_batch = 4;   // rounds launched before the first host check
+    // fallback connectivity searches take tag = ++tag_ctr (at most two per
+    // candidate) and mark cells with (tag << 3) | colour; tag_bound bounds the
+    // counter from the host so the marks are cleared before it reaches 2^27
+    long long tag_bound = 0;
+    void reserve_tags(long long n) {
+        if (tag_bound + n >= (1LL << 27)) {
+            fill<int32_t>(mark, N, 0);
+            set_int(tagctr.as<int>(), 0);
+            tag_bound = 0;
+        }
+        tag_bound += n;
+    }
     DevBuf count, unres, ulist, utgt, utgg, uflag;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     DevBuf E_el, eps_a, eps_b, eps_c;   // elastic energy density and eigenstrain work arrays
@@ -453,6 +465,7 @@ struct Engine final : EngineBase {
         fill<R>(g_comp, N, R(0));
         HIPCHK(hipMemsetAsync(scal.p, 0, sizeof(Scalars), stream));
         HIPCHK(hipMemsetAsync(tagctr.p, 0, 16, stream));
+        tag_bound = 0;
         nlab = 1;
         count.alloc(size_t(nlab + 1) * 4);
         unres.alloc(size_t(nlab + 1));
@@ -486,10 +499,12 @@ struct Engine final : EngineBase {
         k_label_hist<<<nblk((N + 63) / 64, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), count.as<int>());
     }
 
-    void check_labels() {
+    // range of the labels at src (N int32 on the device); a label outside
+    // [0, 2^26) is rejected before anything sized by the label range is touched
+    void check_labels(const int32_t* src) {
         int init[2] = {INT_MAX, INT_MIN};
         HIPCHK(hipMemcpyAsync(minmax.p, init, 8, hipMemcpyHostToDevice, stream));
-        k_minmax_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), minmax.as<int>());
+        k_minmax_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, src, minmax.as<int>());
         int mm[2];
         HIPCHK(hipMemcpyAsync(mm, minmax.p, 8, hipMemcpyDeviceToHost, stream));
         HIPCHK(hipStreamSynchronize(stream));
@@ -503,7 +518,6 @@ struct Engine final : EngineBase {
             unres.alloc(size_t(nlab));
         }
         nlab = std::max(nlab, need);
-        rebuild_counts();
     }
 
     // ---- field access ----
@@ -535,13 +549,19 @@ struct Engine final : EngineBase {
         DevBuf* b = field_buf(f, kind);
         if (!b || kind == 3) throw GpkError(GPK_E_INVALID, "field is not writable");
         if (!src || bytes != size_t(N) * elem(kind)) throw GpkError(GPK_E_INVALID, "upload size mismatch");
-        HIPCHK(hipMemcpyAsync(b->p, src, bytes, device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                              stream));
-        mask_fresh = false;
+        const hipMemcpyKind mk = device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
         if (f == GPK_FIELD_PHI) {
-            check_labels();
+            // labels are staged and range-checked first: on GPK_E_LABEL_RANGE
+            // the grid keeps its previous labels, counts and label tables
+            HIPCHK(hipMemcpyAsync(work4.p, src, bytes, mk, stream));
+            check_labels(work4.as<int32_t>());
+            HIPCHK(hipMemcpyAsync(phi.p, work4.p, bytes, hipMemcpyDeviceToDevice, stream));
+            rebuild_counts();
             rs_reset = true;   // labels written from outside: no cached remap score holds
+        } else {
+            HIPCHK(hipMemcpyAsync(b->p, src, bytes, mk, stream));
         }
+        mask_fresh = false;
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -1079,6 +1099,7 @@ struct Engine final : EngineBase {
             fc.pos_new = pos_new.as<int>(); fc.count = count.as<int>();
             fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
             fc.qflags = qflags.as<int>(); fc.s = ds();
+            reserve_tags(2LL * m);
             int round = 0;
             const int bfs_grid = std::max(1, std::min(m, 512));
             // the first batch is as long as the last step's transaction: rounds
@@ -1372,6 +1393,7 @@ struct Engine final : EngineBase {
         fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>(); fc.pos_new = pos_new.as<int>();
         fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
         fc.qflags = qflags.as<int>(); fc.s = ds();
+        reserve_tags(2);
         k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
         k_flip_bfs<<<1, 256, 0, stream>>>(fc, 0, fbq.as<int>(), mark.as<int>(), reinterpret_cast<int*>(work3.p),
                                           reinterpret_cast<int*>(work4.p), tagctr.as<int>());
@@ -1482,7 +1504,8 @@ struct Engine final : EngineBase {
         HIPCHK(hipGetLastError());
         normalise_device(n_seeds, S);
         fill<R>(alpha, N, R(1));
-        check_labels();
+        check_labels(phi.as<int32_t>());
+        rebuild_counts();
         HIPCHK(hipStreamSynchronize(stream));
     }
 };

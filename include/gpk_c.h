@@ -284,9 +284,13 @@ int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_
  * GPKGrid::step per launch, each with its own dt and time.  Fields are
  * uploaded / downloaded packed (instance 0's cells first, each instance in
  * its own z-y-x order) for GPK_FIELD_PHI, ALPHA, G, C, T, MASK, PHI_COMP,
- * G_COMP.  Scope: D = 0, no grain-network topology test, no solute rejection,
- * no noise (e.g. ANISO_OSTWALD with c = c_sat, TENSOR_ANISO); labels in
- * [0, 1024).  Other models return GPK_E_UNSUPPORTED_MODEL. */
+ * G_COMP.  Scope: D = 0; no solute rejection, noise, stored-energy
+ * (dislocation), precipitation or elastic terms (e.g. ISOTROPIC,
+ * ANISO_GROWTH, RECRYSTALLISATION, ANISO_OSTWALD with c = c_sat,
+ * TENSOR_ANISO); grain-network models (GPK_FLAG_GRAIN_NETWORK) run the
+ * topology test and the extinction of unresolved grains per instance;
+ * labels in [0, 1024) per instance (else GPK_E_LABEL_RANGE).  Other models
+ * return GPK_E_UNSUPPORTED_MODEL. */
 typedef struct gpk_batch gpk_batch;
 
 typedef struct gpk_batch_stats {

@@ -173,6 +173,8 @@ class GPKGrid {
         check(gpk_step(h_, dt, dt_cap, &used, &st));
         time = st.time;
         last_time_ = time;
+        stale_ = true;   // the device moved on: host mirrors are out of date
+        if (sync_each_step) sync_host();
         topology_accepted_last = uint64_t(st.accepted);
         topology_deferred_last = uint64_t(st.deferred);
         gaussian_remap_exchanges_last = uint64_t(st.exchanges);
@@ -250,15 +252,27 @@ class GPKGrid {
     }
 
     // ---- host <-> device ----
+    // The device owns the state.  After step() the host mirrors are stale
+    // until sync_host(); reading them before that sees the previous state.
+    // Callers that edit mirrors call sync_host() first, edit, then
+    // mark_dirty(): the next device call uploads them.  Marking stale mirrors
+    // dirty would upload old values over newer device state, so the upload
+    // refuses it (std::logic_error).  sync_each_step = true downloads every
+    // field after each step (drivers written for host-resident fields).
+    bool sync_each_step = false;
     void mark_dirty() { dirty_ = true; }
-    // Uploads the host mirrors now (phi, alpha, c, T, g, rho).
+    bool host_stale() const { return stale_; }
+    // Uploads the host mirrors now (phi, alpha, c, T, g, rho), whatever the
+    // device holds.
     void upload() const {
         dirty_ = true;
+        stale_ = false;
         push();
     }
     // Downloads every field into the host mirrors.
     void sync_host() const {
         push();
+        stale_ = false;
         auto& self = const_cast<GPKGrid&>(*this);
         get(GPK_FIELD_PHI, self.phi);
         get_real(GPK_FIELD_ALPHA, self.alpha);
@@ -292,6 +306,7 @@ class GPKGrid {
     gpk_handle* h_ = nullptr;
     std::shared_ptr<gpk_handle> handle_;
     mutable bool dirty_ = true;
+    mutable bool stale_ = false;
     mutable double last_time_ = 0.0;
 
     static void check(int rc) {
@@ -312,6 +327,10 @@ class GPKGrid {
             last_time_ = time;
         }
         if (!dirty_) return;
+        if (stale_)
+            throw std::logic_error(
+                "GPKGrid: host fields were marked dirty after a step without sync_host(); "
+                "uploading them would overwrite newer device state (call sync_host() before editing)");
         if (precision_ == GPK_PREC_FP64) {
             // the FP64 engine holds doubles: widen the float mirrors
             put(GPK_FIELD_PHI, phi);

@@ -46,6 +46,30 @@ def test_label_range_is_checked():
     assert e.value.code == _abi.GPK_E_LABEL_RANGE
 
 
+def test_rejected_labels_leave_the_grid_unchanged():
+    # a rejected upload (negative or >= 2^26 labels) keeps the previous labels,
+    # and the grid still steps exactly as the oracle from those labels
+    phi, g = O.voronoi(1, 48, 48, 20, 3)
+    m = _abi.isotropic(1.0, 1.0)
+    G = gpk_amd.GPKGrid(1, 48, 48, 1.0, m, 0.0, 1.5, precision="fp64")
+    o = O.OracleGrid(1, 48, 48, 1.0, m, 0.0, 1.5, "fp64")
+    for X in (G, o):
+        X.set("phi", phi)
+        X.set("g", g)
+    for badval in (-5, 1 << 26, (1 << 31) - 1):
+        bad = phi.reshape(48, 48).copy()
+        bad[7, 9] = badval
+        with pytest.raises(gpk_amd.GPKError) as e:
+            G.set("phi", bad)
+        assert e.value.code == _abi.GPK_E_LABEL_RANGE
+        assert np.array_equal(G.get("phi").ravel(), phi)
+    for _ in range(5):
+        assert G.step() == o.step()
+    assert np.array_equal(G.get("phi"), o.get("phi"))
+    assert G.n_grains() == o.n_grains()
+    G.close()
+
+
 def test_size_mismatch_is_an_error():
     L = _lib.lib()
     G = gpk_amd.GPKGrid(1, 8, 8, 1.0, _abi.isotropic(1.0, 1.0))

@@ -91,3 +91,30 @@ def test_solidification_128_mass_conservation():
     assert abs(G.solute_sum() - M0) <= 1e-13 * M0
     assert G.solid_fraction() > (phi.sum() / phi.size) * 0.5
     G.close()
+
+
+def test_fp64_labels_vs_strict32_128():
+    # P2: plain FP64 against the reference's float labels.  GPU strict32 is
+    # bit-exact with the compiled reference (oracle strict32 == reference,
+    # GPU == oracle), so GPU fp64 against GPU strict32 counts how far FP64
+    # labels move from the reference's at the 256^3 grain density
+    # (128^3, 2,500 grains, sigma 1.5, 50 steps)
+    n, grains = 128, 2500
+    grids = {}
+    for prec in ("strict32", "fp64"):
+        print("128^3 P2", prec, flush=True)
+        G = gpk_amd.GPKGrid(n, n, n, 1.0, _abi.isotropic(1.0, 1.0), 0.0, 1.5, precision=prec)
+        G.voronoi_init(grains, 42)
+        grids[prec] = G
+    assert np.array_equal(grids["strict32"].get("phi"), grids["fp64"].get("phi"))
+    diffs = {}
+    for s in range(1, 51):
+        for G in grids.values():
+            G.step()
+        if s in (1, 10, 25, 50):
+            diffs[s] = int(np.count_nonzero(grids["strict32"].get("phi") != grids["fp64"].get("phi")))
+    print("fp64 vs strict32 label differences by step:", diffs, flush=True)
+    for G in grids.values():
+        G.close()
+    assert diffs[1] == 0 and diffs[10] == 0
+    assert diffs[50] <= 1e-4 * n ** 3
