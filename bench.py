@@ -48,6 +48,10 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# CPU-baseline thread placement (one thread per core, kept close), set before
+# any OpenMP runtime is loaded; the thread count is the environment's
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 import numpy as np  # noqa: E402
 
@@ -95,28 +99,57 @@ def run_gpu(prec, size, grains, sigma, warmup, steps):
     }
 
 
-def kappa_error(size, grains, sigma):
-    """FP32 vs FP64 curvature on identical labels (the initial Voronoi map)."""
+def kappa_error_on(phi, size, sigma):
+    """FP32 vs FP64 curvature on identical labels: max and RMS of
+    |k32 - k64| / max(|k64|, 1e-3 / (sigma dx)) over the band."""
     import gpk_amd
     from gpk_amd import _abi
 
     model = _abi.isotropic(1.0, 1.0)
+    out = {}
     G64 = gpk_amd.GPKGrid(size, size, size, 1.0, model, 0.0, sigma, precision="fp64")
-    G64.voronoi_init(grains, 42)
-    phi = G64.get("phi")
+    G64.set("phi", phi)
+    G64.step(0.0)
+    k64 = G64.get("kappa").ravel()
+    band = G64.get("mask").ravel().astype(bool)
+    G64.close()
     G32 = gpk_amd.GPKGrid(size, size, size, 1.0, model, 0.0, sigma, precision="fp32")
     G32.set("phi", phi)
-    G64.step(0.0)
     G32.step(0.0)
-    k64 = G64.get("kappa").ravel()
     k32 = G32.get("kappa").ravel().astype(np.float64)
-    band = G64.get("mask").ravel().astype(bool)
+    G32.close()
     floor = 1e-3 / sigma
     rel = np.abs(k32[band] - k64[band]) / np.maximum(np.abs(k64[band]), floor)
-    G64.close()
-    G32.close()
-    return {"kappa_rel_err_max": float(rel.max()), "kappa_rel_err_rms": float(np.sqrt(np.mean(rel ** 2))),
-            "kappa_floor": floor}
+    out = {"kappa_rel_err_max": float(rel.max()), "kappa_rel_err_rms": float(np.sqrt(np.mean(rel ** 2))),
+           "kappa_floor": floor}
+    return out
+
+
+def run_full(prec, size, grains, sigma, steps):
+    """The whole configured run (steps from the Voronoi state), one step per
+    call so that every step's band size is recorded."""
+    import gpk_amd
+    from gpk_amd import _abi
+
+    G = gpk_amd.GPKGrid(size, size, size, 1.0, _abi.isotropic(1.0, 1.0), 0.0, sigma, precision=prec)
+    G.voronoi_init(grains, 42)
+    G.sync()
+    bands = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        G.step()
+        bands.append(int(G.last_stats.band_cells))
+    G.sync()
+    wall = time.perf_counter() - t0
+    phi = G.get("phi")
+    G.close()
+    b = np.array(bands, np.float64)
+    return phi, {
+        "precision": prec, "steps": steps, "seconds": wall, "ms_per_step": 1e3 * wall / steps,
+        "cell_updates_per_s": float(b.sum()) / wall,
+        "band_cells_min": int(b.min()), "band_cells_max": int(b.max()),
+        "band_fraction_first": float(b[0]) / size ** 3, "band_fraction_last": float(b[-1]) / size ** 3,
+    }
 
 
 def band_size(phi):
@@ -128,8 +161,28 @@ def band_size(phi):
     return int(np.count_nonzero(m))
 
 
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    threads = None
+    try:
+        threads = int(C.CDLL("libgomp.so.1").omp_get_max_threads())
+    except OSError:
+        pass
+    return model, threads
+
+
 def cpu_baseline(phi, g, size, sigma, cpu_steps):
-    """Reference CPU step (compiled reference sources) on this host."""
+    """Reference CPU step (compiled reference sources) on this host: one
+    untimed warm-up step (first touch of the reference's work arrays), then
+    cpu_steps timed steps, each bracketed by a steady clock."""
     from gpk_amd import _abi
     from oracle import oracle as O
 
@@ -141,6 +194,7 @@ def cpu_baseline(phi, g, size, sigma, cpu_steps):
         R = O.OracleGrid(size, size, size, 1.0, model, 0.0, sigma, "strict32")
     R.set("phi", phi)
     R.set("g", g)
+    R.step()
     band = 0
     wall = 0.0
     for _ in range(cpu_steps):
@@ -148,10 +202,14 @@ def cpu_baseline(phi, g, size, sigma, cpu_steps):
         t0 = time.perf_counter()
         R.step()
         wall += time.perf_counter() - t0
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cpu_model, threads = host_cpu()
+    if threads is None:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": band / wall, "unit": "interface cell-updates/s", "cores": threads,
-            "kind": kind, "seconds": wall,
-            "sample": "%d reference step(s) of the same 256^3 Voronoi state from step 0" % cpu_steps}
+            "kind": kind, "seconds": wall, "cpu_model": cpu_model, "nproc": os.cpu_count(),
+            "omp_proc_bind": os.environ.get("OMP_PROC_BIND"), "omp_places": os.environ.get("OMP_PLACES"),
+            "sample": "reference steps 2-%d of the same %d^3 Voronoi state (step 1 untimed warm-up)"
+                      % (cpu_steps + 1, size)}
 
 
 def solidification_setup(N, rng_seed=7, c0=0.02, k=0.14, undercooling=0.5, r=6):
@@ -403,7 +461,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--grains", type=int, default=20000)
     ap.add_argument("--sigma", type=float, default=1.5)
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--full-steps", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -425,7 +484,6 @@ def main():
         return
 
     G64, r64 = run_gpu("fp64", args.size, args.grains, args.sigma, args.warmup, args.steps)
-    phi0 = None
     res32 = None
     if not args.no_fp32:
         G32, res32 = run_gpu("fp32", args.size, args.grains, args.sigma, args.warmup, args.steps)
@@ -433,9 +491,24 @@ def main():
         p32 = G32.get("phi")
         res32["label_disagreement"] = float(np.count_nonzero(p64 != p32)) / p64.size
         G32.close()
-        res32.update(kappa_error(args.size, args.grains, args.sigma))
         res32["speedup_vs_fp64"] = res32["cell_updates_per_s"] / r64["cell_updates_per_s"]
     G64.close()
+
+    # the whole configured run (200 steps by default) per precision: band
+    # size range, throughput, and the FP32 errors at its end
+    full = {}
+    if args.full_steps > 0:
+        ph64, full["fp64"] = run_full("fp64", args.size, args.grains, args.sigma, args.full_steps)
+        if not args.no_fp32:
+            ph32, full["fp32"] = run_full("fp32", args.size, args.grains, args.sigma, args.full_steps)
+            f32 = full["fp32"]
+            f32["label_disagreement_vs_fp64"] = float(np.count_nonzero(ph64 != ph32)) / ph64.size
+            f32["speedup_vs_fp64"] = f32["cell_updates_per_s"] / full["fp64"]["cell_updates_per_s"]
+            f32["kappa_error_at_end"] = kappa_error_on(ph64, args.size, args.sigma)
+    if res32 is not None:
+        from gpk_amd import voronoi_host
+        phv, _ = voronoi_host(args.size, args.size, args.size, args.grains, 42)
+        res32.update(kappa_error_on(phv, args.size, args.sigma))
 
     cpu = None
     if not args.no_cpu:
@@ -444,19 +517,24 @@ def main():
         cpu = cpu_baseline(phi0, g0, args.size, args.sigma, args.cpu_steps)
 
     flops = r64["geometry_flops_per_launch"]
-    ach = flops / (r64["geometry_ms_avg"] * 1e-3) / 1e12 if r64["geometry_ms_avg"] > 0 else 0.0
+    geo_s = r64["geometry_ms_avg"] * 1e-3
+    eff = flops / geo_s / 1e12 if geo_s > 0 else 0.0
     # FP64 operations the separable kernel executes per band cell in 3-D:
     # per window row one add per extra chunk plus 6 FMAs, per window plane 9 FMAs
     W = 2 * (int(np.ceil(3.5 * args.sigma)) + 2) + 1
     nch = (W + 8) // 9
     band_per_launch = flops / (2.0 * W ** 3 * 9)
     exec_flops = band_per_launch * (W * W * (3 * (nch - 1) + 12) + W * 18)
-    exec_rate = exec_flops / (r64["geometry_ms_avg"] * 1e-3) / 1e12 if r64["geometry_ms_avg"] > 0 else 0.0
+    exec_rate = exec_flops / geo_s / 1e12 if geo_s > 0 else 0.0
     traffic = None
+    traffic_src = None
     tpath = os.path.join(ROOT, "profiles", "geometry_fp64_256_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.size == 256 and args.grains == 20000:
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_src = "%s (rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch; %s)" % (
+            tj.get("source"), tj.get("measured", "undated"))
     out = {
         "metric": "interface cell-updates/s, 256^3 grain growth: FP32 vs FP64 GPU vs CPU",
         "value": r64["cell_updates_per_s"],
@@ -469,23 +547,29 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (periodic Voronoi, 20000 sites, seed 42)",
+        "data": "synthetic (periodic Voronoi, %d sites, seed 42)" % args.grains,
         "config": {"workload": "3-D isotropic grain growth, %d^3 periodic grid, %d Voronoi grains, "
                                "sigma %.2f cells, FP64" % (args.size, args.grains, args.sigma),
                    "grid": [args.size] * 3, "grains": args.grains, "sigma_cells": args.sigma,
                    "precision": "fp64", "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "compute": "VALU (vector FP64; the stencil is not a contraction)",
+        # the dominant kernel executes FP64 vector (VALU) arithmetic; the
+        # masked stencil is not a contraction and uses no matrix cores
+        "roofline": {"bound": "valu", "compute": "FP64 vector ALU (VALU), no MFMA",
                      "kernel": "k_geometry_tiled<double,3,8> (separable row-mask tables)",
-                     "achieved": ach, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
-                     "frac": ach / PEAK_TFLOPS["fp64"], "traffic": traffic,
-                     "flops_basis": "reference formulation, 2 x %d^3 x 9 flops per band cell; "
-                                    "frac above 1 is effective" % W,
-                     "flops_per_launch": flops, "avg_launch_ms": r64["geometry_ms_avg"],
-                     "executed_flops_per_launch": exec_flops, "executed_tflops": exec_rate,
-                     "executed_frac": exec_rate / PEAK_TFLOPS["fp64"]},
+                     "achieved": exec_rate, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
+                     "frac": exec_rate / PEAK_TFLOPS["fp64"], "traffic": traffic,
+                     "traffic_source": traffic_src,
+                     "flops_basis": "FP64 operations the kernel executes: per band cell %d window rows x "
+                                    "(6 FMA + %d adds) + %d planes x 9 FMA" % (W * W, 3 * (nch - 1), W),
+                     "executed_flops_per_launch": exec_flops, "avg_launch_ms": r64["geometry_ms_avg"],
+                     "effective": {"tflops": eff, "frac": eff / PEAK_TFLOPS["fp64"],
+                                   "flops_per_launch": flops,
+                                   "basis": "reference dense formulation, 2 x %d^3 x 9 flops per band cell; "
+                                            "above 1 because the separable kernel does not execute it" % W}},
         "cpu_baseline": cpu,
         "fp64": r64,
         "fp32": res32,
+        "full_run": full or None,
     }
     if cpu:
         out["speedup_vs_cpu"] = r64["cell_updates_per_s"] / cpu["value"]

@@ -1101,7 +1101,7 @@ struct Engine final : EngineBase {
             fc.qflags = qflags.as<int>(); fc.s = ds();
             reserve_tags(2LL * m);
             int round = 0;
-            const int bfs_grid = std::max(1, std::min(m, 512));
+            const int bfs_grid = std::max(1, std::min(m, 768));   // three 43 KB workgroups per CU
             // the first batch is as long as the last step's transaction: rounds
             // past the end are cheap no-ops, host checks are not
             int batch = flip_batch;

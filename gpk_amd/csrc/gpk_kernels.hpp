@@ -13,7 +13,10 @@
 
 namespace gpkd {
 
-enum : int { UNDEC = 0, ACCEPT = 1, REJECT = 2, QUEUED = 3 };
+// RETRY: undecided after an inconclusive bounds search; it is queued again
+// only when every earlier transaction on its labels is decided
+enum : int { UNDEC = 0, ACCEPT = 1, REJECT = 2, QUEUED = 3, RETRY = 4 };
+__device__ __forceinline__ bool undecided(int s) { return s != ACCEPT && s != REJECT; }
 
 // LDS written by some lanes of a wavefront and then read by others
 __device__ __forceinline__ void wave_sync() {
@@ -1132,6 +1135,31 @@ __device__ __forceinline__ int32_t view_label(int cell, int k, const int32_t* ph
     return phi[cell];
 }
 
+// Views of label L bounding every outcome of the earlier transactions not
+// yet decided (each can only add its cell to L or remove it from L):
+//   VIEW_UPPER: an undecided earlier transaction into L counts as accepted,
+//               so the cells read as L include every cell of L at time k;
+//   VIEW_LOWER: an undecided earlier transaction out of L counts as
+//               accepted, so every cell read as L is a cell of L at time k.
+// Face connectivity grows with the cell set: branches disconnected in the
+// upper view are disconnected at time k, connected in the lower view are
+// connected at time k.  A transaction decided while a search runs moves a
+// cell from its bound value to its exact one, which keeps both properties.
+enum : int { VIEW_EXACT = 0, VIEW_UPPER = 1, VIEW_LOWER = 2 };
+template <int MODE>
+__device__ __forceinline__ int32_t view_label_b(int cell, int k, int32_t L, const int32_t* phi,
+                                                const int* cand_id, const int* state, const int* Cold,
+                                                const int* Cnew) {
+    const int j = cand_id[cell];
+    if (j >= 0 && j < k) {
+        const int sj = ld_acq(state + j);
+        if (sj == ACCEPT) return Cnew[j];
+        if (MODE == VIEW_UPPER && sj != REJECT && Cnew[j] == L) return L;
+        if (MODE == VIEW_LOWER && sj != REJECT && Cold[j] == L) return -1;
+    }
+    return phi[cell];
+}
+
 struct FlipCtx {
     Dims d;
     int dim3;
@@ -1155,39 +1183,50 @@ __device__ __forceinline__ void decide(FlipCtx& c, int k, int v) {
     atomicSub(&c.s->undecided, 1);
 }
 
-// all earlier transactions touching label L decided?  (label-ordered prefix)
-__device__ __forceinline__ bool label_prefix_ready(const FlipCtx& c, int pos, int32_t L) {
-    for (int q = pos - 1; q >= 0; --q) {
-        const unsigned long long key = c.Ps[q];
-        if (int32_t(uint32_t(key >> 32)) != L) break;
-        if (ld_acq(c.state + int(uint32_t(key))) == UNDEC ||
-            ld_acq(c.state + int(uint32_t(key))) == QUEUED) return false;
-    }
-    return true;
-}
-
-// count of label L as seen by transaction at label-list position pos
-__device__ __forceinline__ int label_count_at(const FlipCtx& c, int pos, int32_t L) {
-    int n = c.count[L];
+// earlier transactions touching label L (the label-ordered prefix of Ps up
+// to pos): count change of the accepted ones, and the undecided ones split
+// into those that would add a cell to L and those that would remove one
+struct LabelPrefix { int delta, add, rem; };
+__device__ __forceinline__ LabelPrefix label_prefix(const FlipCtx& c, int pos, int32_t L) {
+    LabelPrefix r{0, 0, 0};
     for (int q = pos - 1; q >= 0; --q) {
         const unsigned long long key = c.Ps[q];
         if (int32_t(uint32_t(key >> 32)) != L) break;
         const int j = int(uint32_t(key));
-        if (ld_acq(c.state + j) == ACCEPT) n += (c.Cnew[j] == L) ? 1 : -1;
+        const int sj = ld_acq(c.state + j);
+        const bool adds = c.Cnew[j] == L;
+        if (sj == ACCEPT) r.delta += adds ? 1 : -1;
+        else if (sj != REJECT) { if (adds) ++r.add; else ++r.rem; }
     }
-    return n;
+    return r;
+}
+
+// all earlier transactions touching label L decided?
+__device__ __forceinline__ bool label_prefix_ready(const FlipCtx& c, int pos, int32_t L) {
+    for (int q = pos - 1; q >= 0; --q) {
+        const unsigned long long key = c.Ps[q];
+        if (int32_t(uint32_t(key >> 32)) != L) break;
+        if (undecided(ld_acq(c.state + int(uint32_t(key))))) return false;
+    }
+    return true;
 }
 
 // One round of the ordered flip transaction (src/gpk_transitions.cpp:288-313,
-// certificate src/gpk_connectivity.cpp:175-194).  A candidate is decided once
-// every earlier candidate that can change its inputs is decided:
-//   - earlier candidates in its 3^d cube sharing a label (topological numbers);
-//   - earlier candidates sharing the tested label (count, global BFS).
-// Candidates that need a BFS are queued for k_flip_bfs.
+// certificate src/gpk_connectivity.cpp:175-194).  A candidate's topological
+// numbers are taken once every earlier candidate in its 3^d cube sharing
+// one of its labels is decided.  Its count rule and connectivity searches
+// read the labels at its turn in the serial order; while earlier candidates
+// on the tested label are undecided, they are bounded instead:
+//   - the count of the old label lies in [decided - removals, decided +
+//     additions]: a bound above 1 refuses, a bound at most 1 passes;
+//   - a search is queued for k_flip_bfs on the bounding views (flag 4).
+// A bounds search that settles nothing leaves the candidate RETRY, queued
+// again (exactly) once every earlier candidate on its labels is decided.
 __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= c.m) return;
-    if (ld_acq(c.state + k) != UNDEC) return;
+    const int sk = ld_acq(c.state + k);
+    if (sk != UNDEC && sk != RETRY) return;
     const Dims& d = c.d;
     const int cell = c.Ccell[k];
     const int32_t ol = c.Cold[k], nl = c.Cnew[k];
@@ -1210,7 +1249,7 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
                     const int jo = c.Cold[j], jn = c.Cnew[j];
                     const int sj = ld_acq(c.state + j);
                     if (jo == ol || jo == nl || jn == ol || jn == nl) {
-                        if (sj == UNDEC || sj == QUEUED) return;   // not ready
+                        if (undecided(sj)) return;   // not ready
                     }
                     if (sj == ACCEPT) lab = jn;
                 }
@@ -1228,15 +1267,32 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
     const bool need_old = oo != 1;      // count (oo == 0) or BFS (oo > 1)
     const bool need_new = no > 1;
     if (!need_old && !need_new) { decide(c, k, ACCEPT); return; }
-    if (need_old && !label_prefix_ready(c, c.pos_old[k], ol)) return;
-    if (need_new && !label_prefix_ready(c, c.pos_new[k], nl)) return;
+    // a retried candidate waits for its exact turn
+    if (sk == RETRY) {
+        if (need_old && !label_prefix_ready(c, c.pos_old[k], ol)) return;
+        if (need_new && !label_prefix_ready(c, c.pos_new[k], nl)) return;
+    }
+    bool ready = true;
+    LabelPrefix po{0, 0, 0};
+    if (need_old) {
+        po = label_prefix(c, c.pos_old[k], ol);
+        ready = po.add + po.rem == 0;
+    }
+    if (need_new && ready) ready = label_prefix_ready(c, c.pos_new[k], nl);
     if (oo == 0) {
-        const int cnt = label_count_at(c, c.pos_old[k], ol);
-        if (cnt > 1) { decide(c, k, REJECT); return; }
+        // src/gpk_connectivity.cpp:179-185: a vanishing old label passes only
+        // for the label's last cell
+        const int cnt = c.count[ol] + po.delta;
+        if (cnt - po.rem > 1) { decide(c, k, REJECT); return; }
+        if (cnt + po.add > 1) return;            // depends on undecided candidates
         if (!need_new) { decide(c, k, ACCEPT); return; }
     }
-    const int flags = (oo > 1 ? 1 : 0) | (need_new ? 2 : 0);
-    if (atomicCAS(c.state + k, UNDEC, QUEUED) == UNDEC) {
+    int flags = (oo > 1 ? 1 : 0) | (need_new ? 2 : 0);
+    if (!ready) {
+        if (sk == RETRY) return;
+        flags |= 4;
+    }
+    if (atomicCAS(c.state + k, sk, QUEUED) == sk) {
         c.qflags[k] = flags;
         const int qi = atomicAdd(&c.s->qcount[round & 1], 1);
         c.queue[round & 1][qi] = k;
@@ -1272,7 +1328,9 @@ __device__ __forceinline__ uint32_t hslot(int cell) {
     return (uint32_t(cell) * 2654435761u) >> (32 - 12);
 }
 
-// returns 1 connected, 0 disconnected, -1 overflow
+// returns 1 connected, 0 disconnected, -1 overflow; MODE picks the view of
+// the undecided earlier transactions (view_label_b)
+template <int MODE>
 __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32_t L, bool excl,
                                  int k) {
     const Dims& d = c.d;
@@ -1288,7 +1346,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
         faces(d, cell, f);
         int nb = 0;
         for (int q = 0; q < nface; ++q) {
-            if (view_label(f[q], k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+            if (view_label_b<MODE>(f[q], k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
             const int col = nb++;
             const unsigned long long val = ((unsigned long long)(uint32_t)f[q] << 8) | col;
             uint32_t h = hslot(f[q]);
@@ -1360,7 +1418,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
             for (int q = 0; q < nface; ++q) {
                 const int nbc = f[q];
                 if (excl && nbc == cell) continue;
-                if (view_label(nbc, k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                if (view_label_b<MODE>(nbc, k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
                 const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | col;
                 uint32_t h = hslot(nbc);
                 for (int probe = 0; probe < BFS_HCAP; ++probe) {
@@ -1533,15 +1591,37 @@ __global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, int round, int* __r
         const int k = c.queue[round & 1][qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
-        int ok = 1;
-        if (flags & 1) ok = bfs_connected_lds(S, c, cell, c.Cold[k], true, k);
-        if (ok == 1 && (flags & 2)) ok = bfs_connected_lds(S, c, cell, c.Cnew[k], false, k);
-        if (threadIdx.x == 0) {
-            if (ok < 0) {
-                const int p = atomicAdd(&c.s->fbcount, 1);
-                st_rel(fbq + p, k);
-            } else {
-                decide(c, k, ok ? ACCEPT : REJECT);
+        if (!(flags & 4)) {
+            // exact turn: every earlier transaction on the tested labels is decided
+            int ok = 1;
+            if (flags & 1) ok = bfs_connected_lds<VIEW_EXACT>(S, c, cell, c.Cold[k], true, k);
+            if (ok == 1 && (flags & 2)) ok = bfs_connected_lds<VIEW_EXACT>(S, c, cell, c.Cnew[k], false, k);
+            if (threadIdx.x == 0) {
+                if (ok < 0) {
+                    const int p = atomicAdd(&c.s->fbcount, 1);
+                    st_rel(fbq + p, k);
+                } else {
+                    decide(c, k, ok ? ACCEPT : REJECT);
+                }
+            }
+        } else {
+            // bounds: a test fails if its branches are disconnected in the
+            // upper view, passes if connected in the lower view; one failed
+            // test refuses, all passed accept, anything else waits (RETRY)
+            bool fail = false, open = false;
+            for (int t = 0; t < 2 && !fail; ++t) {
+                if (!(flags & (1 << t))) continue;
+                const int32_t L = t == 0 ? c.Cold[k] : c.Cnew[k];
+                const bool excl = t == 0;
+                const int up = bfs_connected_lds<VIEW_UPPER>(S, c, cell, L, excl, k);
+                if (up == 0) { fail = true; break; }
+                if (up < 0) { open = true; continue; }
+                if (bfs_connected_lds<VIEW_LOWER>(S, c, cell, L, excl, k) != 1) open = true;
+            }
+            if (threadIdx.x == 0) {
+                if (fail) decide(c, k, REJECT);
+                else if (!open) decide(c, k, ACCEPT);
+                else st_rel(c.state + k, RETRY);
             }
         }
         __syncthreads();
