@@ -71,6 +71,7 @@ def lib():
     L.gpk_batch_set_launch.argtypes = [H, C.c_int, C.c_int]
     L.gpk_velocity_field.argtypes = [H, C.POINTER(C.c_double), vp]
     L.gpk_velocity_adjoint.argtypes = [H, C.POINTER(C.c_double), vp, C.POINTER(C.c_double)]
+    L.gpk_velocity_adjoint_noalloc.argtypes = [H, C.POINTER(C.c_double), vp, C.POINTER(C.c_double)]
     L.gpk_topology_simple_move.argtypes = [H, C.c_int, C.c_int32, C.POINTER(C.c_int)]
     L.gpk_voronoi_init.argtypes = [H, C.c_int, C.c_uint32]
     L.gpk_voronoi_host.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint32, vp, vp]
@@ -97,7 +98,7 @@ EXPORTS = (
     "gpk_phase_volume", "gpk_n_grains", "gpk_mean_grain_radius", "gpk_solid_fraction",
     "gpk_interface_fraction", "gpk_solute_sum", "gpk_device_bytes", "gpk_topology_simple_move",
     "gpk_voronoi_init", "gpk_voronoi_host", "gpk_sync", "gpk_set_profiling", "gpk_kernel_timing",
-    "gpk_velocity_field", "gpk_velocity_adjoint",
+    "gpk_velocity_field", "gpk_velocity_adjoint", "gpk_velocity_adjoint_noalloc",
     "gpk_batch_create", "gpk_batch_destroy", "gpk_batch_upload", "gpk_batch_download", "gpk_batch_step",
     "gpk_batch_stats_get", "gpk_batch_instances", "gpk_batch_autotune", "gpk_batch_set_launch",
 )

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <queue>
 #include <random>
 #include <stdexcept>
@@ -243,7 +244,13 @@ struct EngineBase {
     virtual int64_t steps() const = 0;
     virtual void velocity_field(const double theta[3], void* vout) = 0;
     virtual void velocity_adjoint(const double theta[3], const void* gvel, double grad[3]) = 0;
+    virtual void velocity_adjoint_noalloc(const double theta[3], const void* gvel, double grad[3]) = 0;
 };
+
+// size of the allocation-free adjoint's static device scratch (below)
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(16) << 20;
+void scratch_init(int device);
+void* scratch_of(int device);
 
 template <class R, bool kDirect>
 struct Engine final : EngineBase {
@@ -329,6 +336,7 @@ struct Engine final : EngineBase {
         proj_r = std::max(1, static_cast<int>(std::ceil(3.f * vel_sigma)));
         remap_r = std::max(1, static_cast<int>(std::ceil(3.f * remap_sigma)));
         broad = dim3 && W.rg > 18;   // src/gpk_grid.cpp:44
+        scratch_init(device);
         allocate();
         HIPCHK(hipEventCreate(&ev0));
         HIPCHK(hipEventCreate(&ev1));
@@ -449,6 +457,10 @@ struct Engine final : EngineBase {
                                                 stream));
         cub_bytes = std::max({b2, b3, b4}) + 256;
         cub_tmp.alloc(cub_bytes);
+        // block counts of the ordered compaction, sized for the whole grid
+        cmp_cnt.alloc(size_t((n + CMP_CHUNK - 1) / CMP_CHUNK) * 4);
+        // full-grid pair keys of the projection when it cannot use the tile
+        if (proj_r != 3) { pkey.alloc(n * 4); pvfull.alloc(n * rs); }
         // initial values (src/gpk_grid.cpp:14-21)
         fill<int32_t>(phi, N, 0);
         fill<uint8_t>(mask, N, 0);
@@ -949,6 +961,41 @@ struct Engine final : EngineBase {
     }
 
     DevBuf adj_h, adj_part, adj_out;
+    // transposed projection and parameter terms of the adjoint over the nb
+    // band cells velocity_chain left: hbuf (nb doubles) and part (3 doubles
+    // per 256 band cells) are written before they are read, outp (3 doubles)
+    // receives the fixed-order sum
+    void adjoint_terms(const double theta[3], const void* gvel, int nb, double* hbuf, double* part,
+                       double* outp, double grad[3]) {
+        const int nblocks = nblk(nb, 256);
+        gpk_model_desc m = model;
+        m.eps4 = float(theta[0]);
+        m.eps6 = float(theta[1]);
+        m.M0 = float(theta[2]);
+        const R* gv = static_cast<const R*>(gvel);
+        if (dim3) {
+            k_adj_h<R, 3><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
+                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                                                        projw.as<double>(), gv, hbuf);
+            k_adj_terms<R, 3><<<nblocks, 256, 0, stream>>>(
+                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
+                phi_comp.as<int32_t>(), projw.as<double>(), gv, hbuf, nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, part);
+        } else {
+            k_adj_h<R, 2><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
+                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
+                                                        projw.as<double>(), gv, hbuf);
+            k_adj_terms<R, 2><<<nblocks, 256, 0, stream>>>(
+                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
+                phi_comp.as<int32_t>(), projw.as<double>(), gv, hbuf, nx.as<R>(), ny.as<R>(),
+                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, part);
+        }
+        k_adj_final<<<1, 256, 0, stream>>>(nblocks, part, outp);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(grad, outp, 24, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
     void velocity_adjoint(const double theta[3], const void* gvel, double grad[3]) override {
         const double th[3] = {theta[0], theta[1], theta[2]};
         const int nb = velocity_chain(th);
@@ -961,32 +1008,40 @@ struct Engine final : EngineBase {
         if (adj_h.bytes < size_t(nb) * 8) adj_h.alloc(size_t(nb) * 8);
         if (adj_part.bytes < size_t(nblocks) * 24) adj_part.alloc(size_t(nblocks) * 24);
         if (!adj_out.p) adj_out.alloc(24);
-        gpk_model_desc m = model;
-        m.eps4 = float(theta[0]);
-        m.eps6 = float(theta[1]);
-        m.M0 = float(theta[2]);
-        const R* gv = static_cast<const R*>(gvel);
-        if (dim3) {
-            k_adj_h<R, 3><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
-                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
-                                                        projw.as<double>(), gv, adj_h.as<double>());
-            k_adj_terms<R, 3><<<nblocks, 256, 0, stream>>>(
-                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
-                phi_comp.as<int32_t>(), projw.as<double>(), gv, adj_h.as<double>(), nx.as<R>(), ny.as<R>(),
-                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, adj_part.as<double>());
-        } else {
-            k_adj_h<R, 2><<<nblocks, 256, 0, stream>>>(dims, proj_r, nb, bandA.as<int>(), mask.as<uint8_t>(),
-                                                        phi.as<int32_t>(), phi_comp.as<int32_t>(),
-                                                        projw.as<double>(), gv, adj_h.as<double>());
-            k_adj_terms<R, 2><<<nblocks, 256, 0, stream>>>(
-                dims, proj_r, nb, bandA.as<int>(), slot.as<int>(), mask.as<uint8_t>(), phi.as<int32_t>(),
-                phi_comp.as<int32_t>(), projw.as<double>(), gv, adj_h.as<double>(), nx.as<R>(), ny.as<R>(),
-                nz.as<R>(), kappa.as<R>(), g.as<R>(), m, adj_part.as<double>());
+        adjoint_terms(th, gvel, nb, adj_h.as<double>(), adj_part.as<double>(), adj_out.as<double>(), grad);
+    }
+
+    // The same adjoint with its scratch in the library's static device array
+    // (layout: result at 0, partials from 256, terms after the partials,
+    // each 256-byte aligned).  Nothing is allocated: the handle's buffers are
+    // sized at creation, and the per-label separable geometry of wide 3-D
+    // stencils (which sizes its buffers per call) is refused.
+    void velocity_adjoint_noalloc(const double theta[3], const void* gvel, double grad[3]) override {
+        if (broad)
+            throw GpkError(GPK_E_UNSUPPORTED_MODEL,
+                           "allocation-free adjoint: 3-D geometry radius above 18 is not supported");
+        unsigned char* base = static_cast<unsigned char*>(scratch_of(device));
+        if (!base) throw GpkError(GPK_E_HIP, "allocation-free adjoint: scratch not set up on this device");
+        const double th[3] = {theta[0], theta[1], theta[2]};
+        const int nb = velocity_chain(th);
+        grad[0] = grad[1] = grad[2] = 0.0;
+        if (nb == 0) {
+            HIPCHK(hipStreamSynchronize(stream));
+            return;
         }
-        k_adj_final<<<1, 256, 0, stream>>>(nblocks, adj_part.as<double>(), adj_out.as<double>());
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(grad, adj_out.p, 24, hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
+        const size_t nblocks = size_t(nblk(nb, 256));
+        const size_t off_part = 256;
+        const size_t off_h = off_part + (nblocks * 24 + 255) / 256 * 256;
+        const size_t need = off_h + size_t(nb) * 8;
+        if (need > NOALLOC_SCRATCH_BYTES) {
+            HIPCHK(hipStreamSynchronize(stream));
+            throw GpkError(GPK_E_INVALID, "allocation-free adjoint: " + std::to_string(nb) +
+                                              " band cells need " + std::to_string(need) +
+                                              " bytes of scratch, more than its " +
+                                              std::to_string(NOALLOC_SCRATCH_BYTES));
+        }
+        adjoint_terms(th, gvel, nb, reinterpret_cast<double*>(base + off_h),
+                      reinterpret_cast<double*>(base + off_part), reinterpret_cast<double*>(base), grad);
     }
 
     // ---- the step (src/gpk_simulation.cpp:31-54) ----
@@ -1510,6 +1565,34 @@ struct Engine final : EngineBase {
     }
 };
 
+// ---- scratch of the allocation-free adjoint (gpk_velocity_adjoint_noalloc) ----
+// The adjoint's per-band-cell terms (nb doubles), per-workgroup partial sums
+// (3 doubles per 256 band cells) and its 3-value result.  Every GPU the
+// library runs on holds one copy until the process ends.  Host code uses
+// the per-device address from hipGetSymbolAddress, never the array's name.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mu;
+void* scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+// once per device, with that device current (called from the engine's set-up)
+void scratch_init(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES)
+        throw GpkError(GPK_E_INVALID, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (scratch_dev[device]) return;
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw GpkError(GPK_E_HIP, "no device address for the adjoint scratch");
+    scratch_dev[device] = p;
+}
+
+void* scratch_of(int device) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    return scratch_dev[device];
+}
+
 EngineBase* make_engine(const gpk_grid_desc& gd, const gpk_model_desc& m) {
     switch (gd.precision) {
     case GPK_PREC_STRICT32: return new Engine<float, true>(gd, m);
@@ -1643,6 +1726,14 @@ int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_
         NEED_H;
         if (!theta || !gvel_dev || !grad) throw GpkError(GPK_E_INVALID, "null theta, gradient or output");
         h->eng->velocity_adjoint(theta, gvel_dev, grad);
+    });
+}
+
+int gpk_velocity_adjoint_noalloc(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]) {
+    return guarded([&] {
+        NEED_H;
+        if (!theta || !gvel_dev || !grad) throw GpkError(GPK_E_INVALID, "null theta, gradient or output");
+        h->eng->velocity_adjoint_noalloc(theta, gvel_dev, grad);
     });
 }
 

@@ -275,6 +275,16 @@ int gpk_kernel_timing(gpk_handle* h, int which, double* avg_ms, int64_t* launche
  * sure gvel_dev is complete (and vout_dev unused) before the call. */
 int gpk_velocity_field(gpk_handle* h, const double theta[3], void* vout_dev);
 int gpk_velocity_adjoint(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]);
+/* gpk_velocity_adjoint without allocation.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels and result as gpk_velocity_adjoint, bit for bit; the scratch is a
+ * static 16 MiB device array per GPU (held until the process ends), enough
+ * for about 2.09 million band cells: a larger band returns GPK_E_INVALID, a
+ * 3-D grid with geometry radius above 18 GPK_E_UNSUPPORTED_MODEL.  Calls on
+ * one GPU share that array: calls that overlap in time on one GPU (several
+ * host threads, or handles on different streams) race. */
+int gpk_velocity_adjoint_noalloc(gpk_handle* h, const double theta[3], const void* gvel_dev, double grad[3]);
 
 /* ---- Batched 2-D instances ------------------------------------------------
  * No counterpart in the reference, which steps the grids of a sweep one
