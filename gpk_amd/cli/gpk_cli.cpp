@@ -4,7 +4,8 @@
 // [run] model = grain_growth (run_grain_growth, 195-282),
 // [run] model = solidification (run_solidification, 288-391),
 // [run] model = recrystallisation (run_recrystallisation, 477-555) and
-// [run] model = precipitation (run_precipitation, 703-797), with the same
+// [run] model = precipitation (run_precipitation, 703-797) and
+// [run] model = precipitate_breakup (run_precipitate_breakup, 1065-1275), with the same
 // deck keys, initial conditions, progress tables and snapshot files
 // ([output] format = csv | vtk | ppm | all, dir, prefix, write_initial).
 // Optional keys: [gpu] precision = strict32 | fp32 | fp64, [gpu] device.
@@ -620,6 +621,213 @@ int run_precipitation(const Deck& cfg) {
     return 0;
 }
 
+// ---- precipitate break-up sweep (demos/gpk.cpp:1065-1275) ----
+// Per spacing d of [breakup] d_list: a periodic slab of thickness h (z),
+// split into grains 1 = [0, d) and 2 = [d, 2d) along x (Nx = 2d, two grain
+// boundaries per period), beta (label 0) above and below; pure capillary
+// flow (AnisoOstwaldThermo at c = c_sat, D = 0).  The run breaks when the
+// alpha cells (label != 0) fall apart into at least two pieces of
+// min_piece cells or more.  The labels are read between steps, so the host
+// mirrors are refreshed from the device first.
+
+// periodic face-connected pieces of the alpha phase with at least min_size
+// cells (demos/gpk.cpp:975-1002)
+int alpha_pieces(const GPKGrid& g, int min_size) {
+    const int Nz = g.Nz, Ny = g.Ny, Nx = g.Nx, N = g.N();
+    std::vector<uint8_t> seen(size_t(N), 0);
+    std::vector<int> queue;
+    queue.reserve(size_t(N));
+    int big = 0;
+    for (int s = 0; s < N; ++s) {
+        if (g.phi[size_t(s)] == 0 || seen[size_t(s)]) continue;
+        queue.clear();
+        queue.push_back(s);
+        seen[size_t(s)] = 1;
+        for (size_t head = 0; head < queue.size(); ++head) {
+            int z, y, x;
+            g.decode(queue[head], z, y, x);
+            const int nb[6] = {g.idx(z, y, x == 0 ? Nx - 1 : x - 1), g.idx(z, y, x == Nx - 1 ? 0 : x + 1),
+                               g.idx(z, y == 0 ? Ny - 1 : y - 1, x), g.idx(z, y == Ny - 1 ? 0 : y + 1, x),
+                               g.idx(z == 0 ? Nz - 1 : z - 1, y, x), g.idx(z == Nz - 1 ? 0 : z + 1, y, x)};
+            for (int j : nb)
+                if (g.phi[size_t(j)] != 0 && !seen[size_t(j)]) {
+                    seen[size_t(j)] = 1;
+                    queue.push_back(j);
+                }
+        }
+        if (int(queue.size()) >= min_size) ++big;
+    }
+    return big;
+}
+
+// label slice as comma-separated rows: x-z at y = Ny/2 (rows z) or x-y at
+// z = Nz/2 (rows y) (demos/gpk.cpp:1006-1030)
+void label_slice_csv(const GPKGrid& g, const std::string& path, bool xz) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return;
+    const int rows = xz ? g.Nz : g.Ny;
+    for (int r = 0; r < rows; ++r)
+        for (int x = 0; x < g.Nx; ++x) {
+            const int i = xz ? g.idx(r, g.Ny / 2, x) : g.idx(g.Nz / 2, r, x);
+            std::fprintf(f, "%d%c", g.phi[size_t(i)], x == g.Nx - 1 ? '\n' : ',');
+        }
+    std::fclose(f);
+}
+
+// whole grid, one row per cell, in the microstructure CSV layout
+// (demos/gpk.cpp:1036-1062)
+void grid_csv(const GPKGrid& g, const std::string& path, double t, int snap) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) return;
+    std::vector<char> buf(size_t(8) << 20);
+    std::setvbuf(f, buf.data(), _IOFBF, buf.size());
+    std::fprintf(f, "# GPK Microstructure CSV\n# snap=%d  time=%g\n# Nz=%d  Ny=%d  Nx=%d\n# dx=%g [m]\n", snap, t,
+                 g.Nz, g.Ny, g.Nx, double(g.dx));
+    std::fprintf(f, "# cols: x[m] y[m] z[m] grain_id alpha composition temperature orientation\n");
+    const double h = double(g.dx);
+    for (int z = 0; z < g.Nz; ++z)
+        for (int y = 0; y < g.Ny; ++y)
+            for (int x = 0; x < g.Nx; ++x) {
+                const size_t i = size_t(g.idx(z, y, x));
+                std::fprintf(f, "%.6e %.6e %.6e %d %.6f %.6f %.2f %.6f\n", x * h, y * h, z * h, g.phi[i],
+                             g.alpha[i], g.c[i], g.T[i], g.g[i]);
+            }
+    std::fclose(f);
+}
+
+int run_precipitate_breakup(const Deck& cfg) {
+    const int Ny = cfg.i("grid", "Ny", 96), Nz = cfg.i("grid", "Nz", 48);
+    const float dx = cfg.f("grid", "dx", 5.0e-9f);
+    const int nsnap = cfg.i("time", "n_snaps", 30);
+    const float M0 = cfg.f("thermo", "M0", 1.0e-9f);
+    const float sig_broad = cfg.f("thermo", "sigma_broad", 0.091f);
+    const float sig_side = cfg.f("thermo", "sigma_side", 0.811f);
+    const float c_sat = cfg.f("thermo", "c_sat", 0.15f);
+    const float Omega = cfg.f("thermo", "Omega", 5.0e8f);
+    const int h = cfg.i("breakup", "h_cells", 16);
+    const int n_steps = cfg.i("breakup", "n_steps", 2000);
+    const float min_frac = cfg.f("breakup", "min_piece_frac", 0.25f);
+    const std::vector<float> d_list = parse_csv_floats(cfg.str("breakup", "d_list"));
+    if (d_list.empty()) {
+        std::fprintf(stderr, "breakup.d_list is empty\n");
+        return 1;
+    }
+    if (Nz < 2) {
+        std::fprintf(stderr, "precipitate_breakup is 3-D: set grid.Nz > 1\n");
+        return 1;
+    }
+    const std::string dir = cfg.str("output", "dir", "output/precipitate_breakup");
+    // sigma(z) = sigma0 sqrt(bzz) = sigma_broad, sigma(x) = sigma(y) = sigma0 = sigma_side
+    const float bzz = (sig_broad / sig_side) * (sig_broad / sig_side);
+    std::printf("=== GPK Precipitate Breakup 3-D (MI355X) ===\n");
+    std::printf("Nz=%d Ny=%d (Nx=2d per d)  |  h=%d cells  |  %d steps per run\n", Nz, Ny, h, n_steps);
+    std::printf("sigma_broad=%.3f  sigma_side=%.3f J/m^2  -> sigma0=%.3f bxx=byy=1 bzz=%.5f\n\n", double(sig_broad),
+                double(sig_side), double(sig_side), double(bzz));
+    auto thermo = std::make_shared<AnisoOstwaldThermo>(M0, sig_side, 1.f, 1.f, bzz, c_sat, Omega);
+    mkdir_p(dir);
+    FILE* sf = std::fopen((dir + "/summary.csv").c_str(), "w");
+    if (!sf) throw std::runtime_error("cannot write " + dir + "/summary.csv");
+    std::fprintf(sf, "d_cells,h_cells,dh,max_pieces,result\n");
+    std::printf("%-8s %-8s %-12s %-10s\n", "d[cell]", "d/h", "max_pieces", "result");
+    double dh_break = 1e30, dh_stable = 0.0;
+    const int snap_every = std::max(1, n_steps / nsnap);
+    const int ts_every = std::max(1, n_steps / 10);
+    auto name = [&](int d, const char* what) {
+        char b[64];
+        std::snprintf(b, sizeof(b), "/d%03d_%s.csv", d, what);
+        return dir + b;
+    };
+    for (const float df : d_list) {
+        const int d = int(std::lround(df));
+        const float dh = float(d) / float(h);
+        GPKGrid grid(Nz, Ny, 2 * d, dx, thermo, 0.f, cfg.f("geometry", "sigma_cells", 2.f), precision_of(cfg),
+                     cfg.i("gpu", "device", 0));
+        std::fill(grid.phi.begin(), grid.phi.end(), 0);
+        std::fill(grid.alpha.begin(), grid.alpha.end(), 1.f);
+        std::fill(grid.g.begin(), grid.g.end(), 0.f);
+        std::fill(grid.c.begin(), grid.c.end(), c_sat);
+        std::fill(grid.T.begin(), grid.T.end(), 973.f);
+        const int zlo = Nz / 2 - h / 2;
+        for (int z = zlo; z < zlo + h; ++z)
+            for (int y = 0; y < Ny; ++y)
+                for (int x = 0; x < 2 * d; ++x) grid.phi[size_t(grid.idx(z, y, x))] = x < d ? 1 : 2;
+        grid.mark_dirty();
+        const long alpha0 = long(std::count_if(grid.phi.begin(), grid.phi.end(), [](int32_t v) { return v != 0; }));
+        const int min_piece = std::max(8, int(std::lround(min_frac * double(alpha0) / 2)));
+        label_slice_csv(grid, name(d, "xz_t0"), true);
+        label_slice_csv(grid, name(d, "xy_t0"), false);
+        grid_csv(grid, name(d, "3d_t0"), 0.0, 0);
+        grid_csv(grid, name(d, "3d_ts0000"), 0.0, 0);
+        // the most pieces seen (and that state), and the last state that
+        // still held 80% of the alpha cells (checked every 4th step)
+        int max_pieces = 1;
+        std::vector<int32_t> best_phi, grooved_phi;
+        double best_time = 0.0, grooved_time = 0.0;
+        const long alpha_floor = long(0.80 * double(alpha0));
+        for (int s = 0; s < n_steps; ++s) {
+            grid.step(-1.f);
+            const bool vol = s % 4 == 0;
+            const bool pieces = (s + 1) % snap_every == 0 || s == n_steps - 1;
+            const bool dump = (s + 1) % ts_every == 0 || s == n_steps - 1;
+            if (dump) grid.sync_host();
+            else if (vol || pieces) grid.sync_labels();
+            if (vol) {
+                const long cur = long(std::count_if(grid.phi.begin(), grid.phi.end(), [](int32_t v) { return v != 0; }));
+                if (cur >= alpha_floor) {
+                    grooved_phi = grid.phi;
+                    grooved_time = grid.time;
+                }
+            }
+            if (pieces) {
+                const int big = alpha_pieces(grid, min_piece);
+                if (big > max_pieces) {
+                    max_pieces = big;
+                    best_phi = grid.phi;
+                    best_time = grid.time;
+                }
+            }
+            if (dump) {
+                char ts[32];
+                std::snprintf(ts, sizeof(ts), "3d_ts%04d", s + 1);
+                grid_csv(grid, name(d, ts), grid.time, s + 1);
+            }
+        }
+        grid.sync_host();
+        label_slice_csv(grid, name(d, "xz_tf"), true);
+        label_slice_csv(grid, name(d, "xy_tf"), false);
+        // the severed state if there was one, else the last grooved state,
+        // else the final one (labels only; the other fields stay final)
+        const std::vector<int32_t> final_phi = grid.phi;
+        const bool have_best = !best_phi.empty(), have_groove = !grooved_phi.empty();
+        const double disp_time = have_best ? best_time : (have_groove ? grooved_time : grid.time);
+        if (have_best) grid.phi = best_phi;
+        else if (have_groove) grid.phi = grooved_phi;
+        label_slice_csv(grid, name(d, "xz_tb"), true);
+        label_slice_csv(grid, name(d, "xy_tb"), false);
+        grid_csv(grid, name(d, "3d_tb"), disp_time, 1);
+        grid.phi = final_phi;
+        const bool broke = max_pieces >= 2;
+        const char* res = broke ? "BREAK" : "stable";
+        std::printf("%-8d %-8.2f %-12d %-10s\n", d, double(dh), max_pieces, res);
+        std::fprintf(sf, "%d,%d,%.3f,%d,%s\n", d, h, double(dh), max_pieces, res);
+        if (broke) dh_break = std::min(dh_break, double(dh));
+        else dh_stable = std::max(dh_stable, double(dh));
+    }
+    std::fclose(sf);
+    std::printf("\n");
+    if (dh_break < 1e29 && dh_stable > 0.0) {
+        std::printf("Largest stable d/h    = %.2f\n", dh_stable);
+        std::printf("Smallest breaking d/h = %.2f\n", dh_break);
+        std::printf("==> Critical aspect ratio (d/h)* ~ %.2f\n", 0.5 * (dh_break + dh_stable));
+    } else if (dh_break < 1e29) {
+        std::printf("All tested ratios broke up; (d/h)* < %.2f\n", dh_break);
+    } else {
+        std::printf("No breakup observed; (d/h)* > %.2f\n", dh_stable);
+    }
+    std::printf("Snapshots + summary written to %s\n", dir.c_str());
+    return 0;
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
@@ -634,8 +842,10 @@ int main(int argc, char** argv) {
         if (model == "solidification") return run_solidification(cfg);
         if (model == "recrystallisation") return run_recrystallisation(cfg);
         if (model == "precipitation") return run_precipitation(cfg);
+        if (model == "precipitate_breakup") return run_precipitate_breakup(cfg);
         std::fprintf(stderr, "Unknown or unsupported model '%s' (this build runs grain_growth, "
-                             "solidification, recrystallisation and precipitation)\n", model.c_str());
+                             "solidification, recrystallisation, precipitation and precipitate_breakup)\n",
+                     model.c_str());
         return 1;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "Error: %s\n", e.what());

@@ -289,6 +289,12 @@ class GPKGrid {
         get_real(GPK_FIELD_G_COMP, self.g_comp);
         get(GPK_FIELD_MASK, self.mask);
     }
+    // Downloads the labels only (phi); the other mirrors keep their state
+    // (and stay stale after a step).
+    void sync_labels() const {
+        push();
+        get(GPK_FIELD_PHI, const_cast<GPKGrid&>(*this).phi);
+    }
     // Labels and orientations from voronoi_init (demos/gpk.cpp:51-189),
     // generated on the device; mirrors are refreshed.
     void voronoi_init(int n_seeds, uint32_t seed) {

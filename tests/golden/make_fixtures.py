@@ -22,7 +22,10 @@ Fixtures (inputs and expected outputs only):
                      snapshot file the reference CLI wrote for it (CSV, VTK and
                      PPM bytes), demos/gpk.cpp run_grain_growth (195-282),
                      run_solidification (288-391), run_recrystallisation (477-555),
-                     run_precipitation (703-797)
+                     run_precipitation (703-797); cli_precipitate_breakup_3d.npz
+                     (run_precipitate_breakup, 1065-1275) keeps the names of all
+                     files and the bytes of the summary, the label slices and
+                     the first, displayed and last full-grid dumps
 """
 import argparse
 import os
@@ -311,7 +314,41 @@ prefix = g3
 }
 
 
-def cli_snapshots(cli, name, deck):
+# the breakup sweep writes a full-grid dump every n_steps / 10 steps; the
+# fixture keeps the summary, every label slice and the initial, displayed and
+# final full dumps byte for byte, and the names of all files
+CLI_BREAKUP = ("cli_precipitate_breakup_3d", """[run]
+model = precipitate_breakup
+[grid]
+Ny = 12
+Nz = 20
+dx = 5.0e-9
+[time]
+n_snaps = 10
+[thermo]
+M0 = 1.0e-9
+sigma_broad = 0.091
+sigma_side = 0.811
+c_sat = 0.15
+Omega = 0.0
+[breakup]
+h_cells = 4
+n_steps = 200
+min_piece_frac = 0.25
+d_list = 4,16
+[geometry]
+sigma_cells = 1.5
+[output]
+dir = {out}
+""")
+
+
+def breakup_kept(fn):
+    return fn == "summary.csv" or "_xz_" in fn or "_xy_" in fn or fn.endswith(("3d_t0.csv", "3d_tb.csv",
+                                                                               "3d_ts0200.csv"))
+
+
+def cli_snapshots(cli, name, deck, keep=None):
     with tempfile.TemporaryDirectory() as tmp:
         out = os.path.join(tmp, "out")
         path = os.path.join(tmp, "deck.ini")
@@ -319,10 +356,12 @@ def cli_snapshots(cli, name, deck):
             f.write(deck.format(out=out))
         subprocess.check_call([cli, path], stdout=subprocess.DEVNULL, cwd=tmp)
         files = sorted(os.listdir(out))
-        blobs = [np.frombuffer(open(os.path.join(out, fn), "rb").read(), np.uint8) for fn in files]
+        blobs = [np.frombuffer(open(os.path.join(out, fn), "rb").read(), np.uint8)
+                 if keep is None or keep(fn) else None for fn in files]
     arrays = {"deck": np.array(deck), "files": np.array(files)}
     for k, b in enumerate(blobs):
-        arrays["f%d" % k] = b
+        if b is not None:
+            arrays["f%d" % k] = b
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
 
 
@@ -338,6 +377,7 @@ def main():
         voronoi(cli, *dims)
     for name, deck in CLI_DECKS.items():
         cli_snapshots(cli, name, deck)
+    cli_snapshots(cli, *CLI_BREAKUP, keep=breakup_kept)
 
 
 if __name__ == "__main__":

@@ -176,3 +176,26 @@ def test_batch_grain_network_larger_instances():
             assert np.array_equal(got[f][b], o.get(f)), (b, f)
     assert deferred > 0 and st["deferred"] == deferred
     B.close()
+
+
+def test_batch_full_scale_lamellar_fp32_guard_band():
+    # config-4 scale: instances of side 300-384 with the sweep's stripe
+    # parameters (t in [6, 20], period ratio in [2, 12]), 150 steps in FP32
+    # with the FP64 guard band, each instance against the oracle's fp32 mode
+    shapes, phis, params = lamellar_sweep(5, seed=11, side_range=(300, 384))
+    steps = 150
+    print("full-scale batch", params, flush=True)
+    B = make_batch(shapes, phis, "fp32")
+    B.step(steps)
+    t, _, _ = B.instances()
+    got = {f: B.get(f) for f in ("phi", "alpha", "g")}
+    st = B.stats()
+    assert st["accepted"] > 0
+    for b, (shape, phi) in enumerate(zip(shapes, phis)):
+        o = oracle_run(shape, phi, "fp32", steps)
+        for f in ("phi", "alpha", "g"):
+            ref = o.get(f)
+            assert np.array_equal(got[f][b], ref), (b, f, int(np.count_nonzero(got[f][b] != ref)))
+        assert t[b] == o.time, (b, t[b], o.time)
+    print("accepted", st["accepted"], "exchanges", st["exchanges"], "guarded", st["guarded"], flush=True)
+    B.close()

@@ -72,3 +72,20 @@ def test_cli_solidification_2d_anisotropic(tmp_path):
             scale = max(np.abs(want[:, col]).max(), 1e-30)
             err = np.abs(got[:, col] - want[:, col]).max() / scale
             assert err <= 1e-4, (fn, col, err)
+
+
+def test_cli_precipitate_breakup_matches_reference(tmp_path):
+    # the sweep reads the labels between steps (volume every 4th step, pieces
+    # every n_steps / n_snaps), so this also checks the facade's host sync;
+    # d = 4 stays one piece, d = 16 severs (summary: stable, BREAK)
+    fx, out, files = run_deck("cli_precipitate_breakup_3d", tmp_path)
+    kept = 0
+    for k, fn in enumerate(files):
+        if "f%d" % k not in fx.files:
+            continue
+        want = fx["f%d" % k].tobytes()
+        got = (out / fn).read_bytes()
+        assert got == want, (fn, first_difference(got, want))
+        kept += 1
+    assert kept >= 15
+    assert (out / "summary.csv").read_text().splitlines()[1:] == ["4,4,1.000,1,stable", "16,4,4.000,2,BREAK"]
