@@ -6,7 +6,6 @@
 // counts that size the next launch (band size, candidates, proposals) and
 // the undecided count of the ordered transactions.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <atomic>
@@ -29,6 +28,7 @@
 #include "gpk_adjoint.hpp"
 #include "gpk_broad.hpp"
 #include "gpk_host.hpp"
+#include "gpk_sort.hpp"
 
 using namespace gpkd;
 using namespace gpkh;
@@ -275,8 +275,10 @@ struct Engine final : EngineBase {
     DevBuf cflag, anew_s, sel;
     DevBuf Ccell, Cold, Cnew, Cg, Canew, state, pos_old, pos_new, qflags, queue0, queue1, fbq;
     DevBuf Pk, Ps;
-    DevBuf gain_s, pcell, kgain, kgain2, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
+    DevBuf gain_s, pcell, kgain, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
         pflag2, pidx;
+    // radix sort: ping-pong keys and payloads, (digit, block) counts, total
+    DevBuf rs_keys, rs_vals, rs_cnt, rs_total;
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt, pkey, pvfull;
     bool mask_fresh = false;    // mask matches the current labels
     int flip_batch = 4, pair_batch = 4;   // rounds launched before the first host check
@@ -301,14 +303,12 @@ struct Engine final : EngineBase {
     bool rs_reset = true;
     bool el_nonzero = false;
     // broad-kernel 3-D geometry (stencil radius > 18): Gaussian taps, label
-    // flags / list of the interface labels, hipcub scratch for the label list
+    // flags / list of the interface labels, per-label boxes and pass buffers
     bool broad = false;
     DevBuf kw, lflag, llist, blidx, bbb, bboxes, bsx, bsy, bsz;
     std::vector<int32_t> hlabels;
     std::vector<BroadBox> hboxes;
     DevBuf scal, dsum, tagctr, minmax, noise_amp;
-    DevBuf cub_tmp;
-    size_t cub_bytes = 0;
     Scalars* hs = nullptr;      // pinned host mirror of the scalar block
     int geo_band = 0;           // size of the geometry band (bandA)
     bool fields_valid = false;
@@ -359,11 +359,11 @@ struct Engine final : EngineBase {
                                &g, &rho, &g_comp, &work1, &work2, &work3, &work4, &bandA, &bandB,
                                &nx, &ny, &nz, &kappa, &vraw, &vn, &cflag, &anew_s, &sel, &Ccell,
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
-                               &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain, &kgain2,
+                               &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain,
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
                                &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &cmp_cnt, &pkey, &pvfull, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
-                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &cub_tmp, &kw, &lflag, &E_el, &eps_a,
+                               &scal, &dsum, &tagctr, &minmax, &noise_amp, &rs_keys, &rs_vals, &rs_cnt, &rs_total, &kw, &lflag, &E_el, &eps_a,
                                &rs_gain, &rs_best, &rs_valid, &rs_dirty,
                                &eps_b, &eps_c,
                                &llist, &blidx, &bbb, &bboxes, &bsx, &bsy, &bsz};
@@ -398,7 +398,7 @@ struct Engine final : EngineBase {
         Canew.alloc(n * rs); state.alloc(n * 4); pos_old.alloc(n * 4); pos_new.alloc(n * 4);
         qflags.alloc(n * 4); queue0.alloc(n * 4); queue1.alloc(n * 4); fbq.alloc(n * 4);
         Pk.alloc(2 * n * 8); Ps.alloc(2 * n * 8);
-        gain_s.alloc(n * rs); pcell.alloc(n * 4); kgain.alloc(n * 8); kgain2.alloc(n * 8);
+        gain_s.alloc(n * rs); pcell.alloc(n * 4); kgain.alloc(n * 8);
         idx.alloc(n * 4); order.alloc(n * 4); kgrp.alloc(n * 8); kgrp2.alloc(n * 8);
         order2.alloc(n * 4); seg_start.alloc(n * 4); seg_end.alloc(n * 4); pflag2.alloc(n * 4);
         pidx.alloc(n * 4);
@@ -446,17 +446,12 @@ struct Engine final : EngineBase {
         projw.alloc(pw.size() * 8); remapw.alloc(rw.size() * 8);
         HIPCHK(hipMemcpy(projw.p, pw.data(), pw.size() * 8, hipMemcpyHostToDevice));
         HIPCHK(hipMemcpy(remapw.p, rw.data(), rw.size() * 8, hipMemcpyHostToDevice));
-        // hipcub scratch: sized for the largest call
-        size_t b2 = 0, b3 = 0, b4 = 0;
-        HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, b2, Pk.as<unsigned long long>(),
-                                                 Ps.as<unsigned long long>(), 2 * N, 0, 64, stream));
-        HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, b3, kgain.as<unsigned long long>(),
-                                                  kgain2.as<unsigned long long>(), idx.as<int>(),
-                                                  order.as<int>(), N, 0, 64, stream));
-        HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, b4, pflag2.as<int>(), pidx.as<int>(), N,
-                                                stream));
-        cub_bytes = std::max({b2, b3, b4}) + 256;
-        cub_tmp.alloc(cub_bytes);
+        // radix sort ping-pong buffers and per-(digit, block) counts, sized for
+        // the largest sort (the 2m flip keys, m <= N)
+        rs_keys.alloc(2 * n * 8);
+        rs_vals.alloc(n * 4);
+        rs_cnt.alloc(size_t(RS_BINS) * size_t((2 * n + RS_CHUNK - 1) / RS_CHUNK) * 4);
+        rs_total.alloc(16);
         // block counts of the ordered compaction, sized for the whole grid
         cmp_cnt.alloc(size_t((n + CMP_CHUNK - 1) / CMP_CHUNK) * 4);
         // full-grid pair keys of the projection when it cannot use the tile
@@ -817,6 +812,40 @@ struct Engine final : EngineBase {
         k_cmp_scatter<<<nb, CMP_T, 0, stream>>>(n, flags, cmp_cnt.as<int>(), out, slot_map);
     }
 
+    // ---- stable LSD radix sort of n 64-bit keys by bits [begin, end), kin
+    // -> kout, int payloads vin -> vout when given (gpk_sort.hpp); kin and
+    // vin are read only, the passes alternate between kout and rs_keys so
+    // that the last one lands in kout
+    void radix_sort(const unsigned long long* kin, unsigned long long* kout, const int* vin, int* vout, int n,
+                    int begin, int end) {
+        if (n <= 0) return;
+        const int passes = (end - begin + 7) / 8;
+        if (passes <= 0) {
+            HIPCHK(hipMemcpyAsync(kout, kin, size_t(n) * 8, hipMemcpyDeviceToDevice, stream));
+            if (vin) HIPCHK(hipMemcpyAsync(vout, vin, size_t(n) * 4, hipMemcpyDeviceToDevice, stream));
+            return;
+        }
+        const int nb = (n + RS_CHUNK - 1) / RS_CHUNK;
+        if (rs_cnt.bytes < size_t(RS_BINS) * nb * 4 || rs_keys.bytes < size_t(n) * 8 ||
+            (vin && rs_vals.bytes < size_t(n) * 4))
+            throw GpkError(GPK_E_INVALID, "radix sort larger than its buffers");
+        const unsigned long long* ks = kin;
+        const int* vs = vin;
+        for (int p = 0; p < passes; ++p) {
+            const int shift = begin + 8 * p;
+            const int bits = std::min(8, end - shift);
+            const bool to_out = (passes - 1 - p) % 2 == 0;
+            unsigned long long* kd = to_out ? kout : rs_keys.as<unsigned long long>();
+            // a keys-only sort (vin null) keeps both payload pointers null
+            int* vd = vin ? (to_out ? vout : rs_vals.as<int>()) : nullptr;
+            k_rs_count<<<nb, RS_T, 0, stream>>>(n, ks, shift, bits, rs_cnt.as<int>());
+            k_cmp_scan<<<1, 1024, 0, stream>>>(RS_BINS * nb, rs_cnt.as<int>(), rs_total.as<int>());
+            k_rs_scatter<<<nb, RS_T, 0, stream>>>(n, ks, kd, vs, vd, shift, bits, rs_cnt.as<int>());
+            ks = kd;
+            vs = vd;
+        }
+    }
+
     // ---- interface mask (src/gpk_geometry.cpp:42-82); mask_fresh records
     // that mask matches the current labels, so a compaction can skip it
     void update_mask() {
@@ -1136,12 +1165,12 @@ struct Engine final : EngineBase {
             anew_s.as<R>(), Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(),
             state.as<int>(), cand_id.as<int>(), network ? Pk.as<unsigned long long>() : nullptr);
         if (network) {
-            size_t tb = cub_bytes;
+            // (label << 32 | k) keys are generated in k order, so a stable sort
+            // on the label bits alone orders them by (label, k)
             int hibit = 32;
             while ((1LL << (hibit - 32)) < nlab) ++hibit;
-            HIPCHK(hipcub::DeviceRadixSort::SortKeys(cub_tmp.p, tb, Pk.as<unsigned long long>(),
-                                                     Ps.as<unsigned long long>(), 2 * m, 0,
-                                                     std::min(64, hibit + 1), stream));
+            radix_sort(Pk.as<unsigned long long>(), Ps.as<unsigned long long>(), nullptr, nullptr, 2 * m, 32,
+                       std::min(64, hibit + 1));
             k_cand_pos<<<nblk(2 * m, 256), 256, 0, stream>>>(2 * m, Ps.as<unsigned long long>(),
                                                               Cold.as<int>(), pos_old.as<int>(),
                                                               pos_new.as<int>());
@@ -1227,38 +1256,30 @@ struct Engine final : EngineBase {
                                                               gain_s.as<R>(), phi.as<int32_t>(),
                                                               phi_comp.as<int32_t>(), pcell.as<int>(),
                                                               kgain.as<unsigned long long>(), idx.as<int>());
-            size_t tb = cub_bytes;
-            HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp.p, tb, kgain.as<unsigned long long>(),
-                                                      kgain2.as<unsigned long long>(), idx.as<int>(),
-                                                      order.as<int>(), np, 0, sizeof(R) == 4 ? 32 : 64,
-                                                      stream));
-            // group keys hold two labels of hb bits each and the direction
+            // order (low, high, dir) ascending, gain descending, cell ascending
+            // (the reference's std::map of per-pair lists sorted by gain, then
+            // cell): a stable radix sort of the proposals (cell order) by the
+            // group key, then a rank sort by gain within each group
             int hb = 1;
             while ((1LL << hb) < nlab) ++hb;
-            k_prop_group_keys<<<nblk(np, 256), 256, 0, stream>>>(np, hb, order.as<int>(), pcell.as<int>(),
+            k_prop_group_keys<<<nblk(np, 256), 256, 0, stream>>>(np, hb, idx.as<int>(), pcell.as<int>(),
                                                                   phi.as<int32_t>(), phi_comp.as<int32_t>(),
                                                                   kgrp.as<unsigned long long>());
-            tb = cub_bytes;
-            HIPCHK(hipcub::DeviceRadixSort::SortPairs(cub_tmp.p, tb, kgrp.as<unsigned long long>(),
-                                                      kgrp2.as<unsigned long long>(), order.as<int>(),
-                                                      order2.as<int>(), np, 0, 2 * hb + 1, stream));
+            radix_sort(kgrp.as<unsigned long long>(), kgrp2.as<unsigned long long>(), idx.as<int>(),
+                       order.as<int>(), np, 0, 2 * hb + 1);
             k_seg_bounds<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
                                                              seg_start.as<int>(), seg_end.as<int>());
+            k_seg_rank<<<nblk(np, 256), 256, 0, stream>>>(np, order.as<int>(), kgain.as<unsigned long long>(),
+                                                           seg_start.as<int>(), seg_end.as<int>(),
+                                                           order2.as<int>());
             k_pair_flags<<<nblk(np, 256), 256, 0, stream>>>(np, kgrp2.as<unsigned long long>(),
                                                              seg_start.as<int>(), seg_end.as<int>(),
-                                                             pflag2.as<int>());
-            tb = cub_bytes;
-            HIPCHK(hipcub::DeviceScan::ExclusiveSum(cub_tmp.p, tb, pflag2.as<int>(), pidx.as<int>(), np,
-                                                    stream));
-            int tail[2];
-            HIPCHK(hipMemcpyAsync(&tail[0], pidx.as<int>() + (np - 1), 4, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipMemcpyAsync(&tail[1], pflag2.as<int>() + (np - 1), 4, hipMemcpyDeviceToHost, stream));
-            HIPCHK(hipStreamSynchronize(stream));
-            npairs = tail[0] + tail[1];
+                                                             pflag2.as<uint8_t>());
+            npairs = select_flags(pflag2.as<uint8_t>(), np, pidx.as<int>());
             if (npairs > 0) {
-                k_pair_emit<R><<<nblk(np, 256), 256, 0, stream>>>(
-                    np, hb, kgrp2.as<unsigned long long>(), seg_start.as<int>(), seg_end.as<int>(),
-                    pflag2.as<int>(), pidx.as<int>(), order2.as<int>(), pcell.as<int>(), Pa.as<int>(),
+                k_pair_emit<R><<<nblk(npairs, 256), 256, 0, stream>>>(
+                    npairs, hb, kgrp2.as<unsigned long long>(), seg_start.as<int>(), seg_end.as<int>(),
+                    pidx.as<int>(), order2.as<int>(), pcell.as<int>(), Pa.as<int>(),
                     Pb.as<int>(), Plo.as<int>(), Phi.as<int>(), pstate.as<int>(), pid.as<int>());
                 set_int(&ds()->undecided, npairs);
                 PairCtx pc;

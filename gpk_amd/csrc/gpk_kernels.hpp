@@ -1750,7 +1750,7 @@ __global__ void k_seg_bounds(int np, const unsigned long long* __restrict__ kg,
 // forward entry at rank r pairs with the reverse entry of the same rank
 __global__ void k_pair_flags(int np, const unsigned long long* __restrict__ kg,
                              const int* __restrict__ seg_start, const int* __restrict__ seg_end,
-                             int* __restrict__ pflag) {
+                             uint8_t* __restrict__ pflag) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= np) return;
     int f = 0;
@@ -1763,22 +1763,23 @@ __global__ void k_pair_flags(int np, const unsigned long long* __restrict__ kg,
             if (rnk <= re - rs) f = 1;
         }
     }
-    pflag[s] = f;
+    pflag[s] = uint8_t(f);
 }
 
+// pair q: the q-th flagged forward entry (plist, in sorted order)
 template <class R>
-__global__ void k_pair_emit(int np, int hb, const unsigned long long* __restrict__ kg,
+__global__ void k_pair_emit(int npairs, int hb, const unsigned long long* __restrict__ kg,
                             const int* __restrict__ seg_start, const int* __restrict__ seg_end,
-                            const int* __restrict__ pflag, const int* __restrict__ pidx,
+                            const int* __restrict__ plist,
                             const int* __restrict__ order, const int* __restrict__ pcell,
                             int* __restrict__ Pa, int* __restrict__ Pb, int* __restrict__ Plo,
                             int* __restrict__ Phi, int* __restrict__ pstate,
                             int* __restrict__ pid) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= np || !pflag[s]) return;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= npairs) return;
+    const int s = plist[q];
     const int rnk = s - seg_start[s];
     const int rs = seg_end[s] + 1;
-    const int q = pidx[s];
     const int a = pcell[order[s]], b = pcell[order[rs + rnk]];
     const unsigned long long key = kg[s] >> 1;
     Pa[q] = a; Pb[q] = b;
