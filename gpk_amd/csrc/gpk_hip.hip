@@ -273,7 +273,7 @@ struct Engine final : EngineBase {
     DevBuf alpha, c, T, g, rho, g_comp, work1, work2, work3, work4;
     DevBuf bandA, bandB, nx, ny, nz, kappa, vraw, vn;
     DevBuf cflag, anew_s, sel;
-    DevBuf Ccell, Cold, Cnew, Cg, Canew, state, pos_old, pos_new, qflags, queue0, queue1, fbq;
+    DevBuf Ccell, Cold, Cnew, Cg, Canew, state, pos_old, pos_new, qflags, queue0, queue1, fbq, ovq;
     DevBuf Pk, Ps;
     DevBuf gain_s, pcell, kgain, idx, order, kgrp, kgrp2, order2, seg_start, seg_end,
         pflag2, pidx;
@@ -359,7 +359,7 @@ struct Engine final : EngineBase {
                                &g, &rho, &g_comp, &work1, &work2, &work3, &work4, &bandA, &bandB,
                                &nx, &ny, &nz, &kappa, &vraw, &vn, &cflag, &anew_s, &sel, &Ccell,
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
-                               &queue0, &queue1, &fbq, &Pk, &Ps, &gain_s, &pcell, &kgain,
+                               &queue0, &queue1, &fbq, &ovq, &Pk, &Ps, &gain_s, &pcell, &kgain,
                                &idx, &order, &kgrp, &kgrp2, &order2, &seg_start, &seg_end, &pflag2,
                                &pidx, &Pa, &Pb, &Plo, &Phi, &pstate, &Pg, &pdeps, &cmp_cnt, &pkey, &pvfull, &count, &unres, &ulist,
                                &utgt, &utgg, &uflag, &wt0, &wt1, &wt2, &tab, &projw, &remapw,
@@ -396,7 +396,7 @@ struct Engine final : EngineBase {
         cflag.alloc(n); anew_s.alloc(n * rs); sel.alloc(n * 4);
         Ccell.alloc(n * 4); Cold.alloc(n * 4); Cnew.alloc(n * 4); Cg.alloc(n * rs);
         Canew.alloc(n * rs); state.alloc(n * 4); pos_old.alloc(n * 4); pos_new.alloc(n * 4);
-        qflags.alloc(n * 4); queue0.alloc(n * 4); queue1.alloc(n * 4); fbq.alloc(n * 4);
+        qflags.alloc(n * 4); queue0.alloc(n * 4); queue1.alloc(n * 4); fbq.alloc(n * 4); ovq.alloc(n * 4);
         Pk.alloc(2 * n * 8); Ps.alloc(2 * n * 8);
         gain_s.alloc(n * rs); pcell.alloc(n * 4); kgain.alloc(n * 8);
         idx.alloc(n * 4); order.alloc(n * 4); kgrp.alloc(n * 8); kgrp2.alloc(n * 8);
@@ -1174,7 +1174,8 @@ struct Engine final : EngineBase {
             k_cand_pos<<<nblk(2 * m, 256), 256, 0, stream>>>(2 * m, Ps.as<unsigned long long>(),
                                                               Cold.as<int>(), pos_old.as<int>(),
                                                               pos_new.as<int>());
-            set_ints({{&ds()->undecided, m}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0}});
+            set_ints({{&ds()->undecided, m}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0},
+                      {&ds()->ovcount[0], 0}, {&ds()->ovcount[1], 0}});
             FlipCtx fc;
             fc.d = dims; fc.dim3 = dim3; fc.m = m;
             fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
@@ -1185,14 +1186,20 @@ struct Engine final : EngineBase {
             fc.qflags = qflags.as<int>(); fc.s = ds();
             reserve_tags(2LL * m);
             int round = 0;
-            const int bfs_grid = std::max(1, std::min(m, 768));   // three 43 KB workgroups per CU
+            // one wavefront per queued search (four per workgroup, three
+            // 43 KB workgroups per CU), then the workgroup search for those
+            // that outgrew a wavefront's hash
+            const int wave_grid = std::max(1, std::min((m + WB_WAVES - 1) / WB_WAVES, 768));
+            const int bfs_grid = std::max(1, std::min(m, 384));
             // the first batch is as long as the last step's transaction: rounds
             // past the end are cheap no-ops, host checks are not
             int batch = flip_batch;
             while (true) {
                 for (int r = 0; r < batch; ++r, ++round) {
                     k_flip_round<<<nblk(m, 256), 256, 0, stream>>>(fc, round);
-                    k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, round, fbq.as<int>(), mark.as<int>(),
+                    k_flip_bfs_wave<<<wave_grid, 256, 0, stream>>>(fc, round, ovq.as<int>(), ds()->ovcount);
+                    k_flip_bfs<<<bfs_grid, 256, 0, stream>>>(fc, ovq.as<int>(), &ds()->ovcount[round & 1],
+                                                             fbq.as<int>(), mark.as<int>(),
                                                              reinterpret_cast<int*>(work3.p),
                                                              reinterpret_cast<int*>(work4.p), tagctr.as<int>());
                 }
@@ -1471,7 +1478,8 @@ struct Engine final : EngineBase {
         fc.qflags = qflags.as<int>(); fc.s = ds();
         reserve_tags(2);
         k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
-        k_flip_bfs<<<1, 256, 0, stream>>>(fc, 0, fbq.as<int>(), mark.as<int>(), reinterpret_cast<int*>(work3.p),
+        k_flip_bfs<<<1, 256, 0, stream>>>(fc, queue0.as<int>(), &ds()->qcount[0], fbq.as<int>(), mark.as<int>(),
+                                          reinterpret_cast<int*>(work3.p),
                                           reinterpret_cast<int*>(work4.p), tagctr.as<int>());
         int st = 0;
         HIPCHK(hipMemcpyAsync(&st, state.p, 4, hipMemcpyDeviceToHost, stream));

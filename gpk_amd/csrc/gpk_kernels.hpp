@@ -1007,6 +1007,7 @@ struct Scalars {
     int ncand;                      // flip candidates
     int undecided;                  // undecided transactions
     int qcount[2];                  // BFS queues (double-buffered per round)
+    int ovcount[2];                 // wavefront searches handed to the workgroup search, per round
     int fbcount;                    // BFS fallback queue
     int accepted, deferred;
     int nprop;                      // remap proposals
@@ -1222,11 +1223,25 @@ __device__ __forceinline__ bool label_prefix_ready(const FlipCtx& c, int pos, in
 //   - a search is queued for k_flip_bfs on the bounding views (flag 4).
 // A bounds search that settles nothing leaves the candidate RETRY, queued
 // again (exactly) once every earlier candidate on its labels is decided.
-__global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= c.m) return;
+constexpr int FLIP_RESPEC = 4;
+
+// a verdict taken where another thread may look at the same candidate: only
+// the thread whose claim succeeds counts it
+__device__ __forceinline__ int decide_claim(FlipCtx& c, int k, int seen, int v) {
+    if (atomicCAS(c.state + k, seen, v) != seen) return 0;
+    atomicAdd(v == ACCEPT ? &c.s->accepted : &c.s->deferred, 1);
+    atomicSub(&c.s->undecided, 1);
+    return 1;
+}
+
+// One look at candidate k in round `round`: 0 nothing (decided already,
+// waiting, or claimed elsewhere), 1 decided here, 2 claimed for a search
+// (state QUEUED, qflags written, *prev = the state it had); the caller runs
+// the search or queues it.
+__device__ int flip_try(FlipCtx& c, int k, int round, int* prev) {
     const int sk = ld_acq(c.state + k);
-    if (sk != UNDEC && sk != RETRY) return;
+    *prev = sk;
+    if (sk != UNDEC && sk != RETRY) return 0;
     const Dims& d = c.d;
     const int cell = c.Ccell[k];
     const int32_t ol = c.Cold[k], nl = c.Cnew[k];
@@ -1249,7 +1264,7 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
                     const int jo = c.Cold[j], jn = c.Cnew[j];
                     const int sj = ld_acq(c.state + j);
                     if (jo == ol || jo == nl || jn == ol || jn == nl) {
-                        if (undecided(sj)) return;   // not ready
+                        if (undecided(sj)) return 0;   // not ready
                     }
                     if (sj == ACCEPT) lab = jn;
                 }
@@ -1263,14 +1278,17 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
     occ_n &= ~(1u << centre);
     const int oo = topo_count(occ_o, c.dim3 != 0);
     const int no = topo_count(occ_n, c.dim3 != 0);
-    if (no == 0) { decide(c, k, REJECT); return; }
+    if (no == 0) return decide_claim(c, k, sk, REJECT);
     const bool need_old = oo != 1;      // count (oo == 0) or BFS (oo > 1)
     const bool need_new = no > 1;
-    if (!need_old && !need_new) { decide(c, k, ACCEPT); return; }
-    // a retried candidate waits for its exact turn
-    if (sk == RETRY) {
-        if (need_old && !label_prefix_ready(c, c.pos_old[k], ol)) return;
-        if (need_new && !label_prefix_ready(c, c.pos_new[k], nl)) return;
+    if (!need_old && !need_new) return decide_claim(c, k, sk, ACCEPT);
+    // a retried candidate tries its bounds again every FLIP_RESPEC rounds
+    // (they narrow as earlier candidates are decided), else waits for its
+    // exact turn; qflags bits 8 and up hold the round of its last search
+    const bool respec = sk == RETRY && round - (c.qflags[k] >> 8) >= FLIP_RESPEC;
+    if (sk == RETRY && !respec) {
+        if (need_old && !label_prefix_ready(c, c.pos_old[k], ol)) return 0;
+        if (need_new && !label_prefix_ready(c, c.pos_new[k], nl)) return 0;
     }
     bool ready = true;
     LabelPrefix po{0, 0, 0};
@@ -1283,17 +1301,25 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
         // src/gpk_connectivity.cpp:179-185: a vanishing old label passes only
         // for the label's last cell
         const int cnt = c.count[ol] + po.delta;
-        if (cnt - po.rem > 1) { decide(c, k, REJECT); return; }
-        if (cnt + po.add > 1) return;            // depends on undecided candidates
-        if (!need_new) { decide(c, k, ACCEPT); return; }
+        if (cnt - po.rem > 1) return decide_claim(c, k, sk, REJECT);
+        if (cnt + po.add > 1) return 0;          // depends on undecided candidates
+        if (!need_new) return decide_claim(c, k, sk, ACCEPT);
     }
     int flags = (oo > 1 ? 1 : 0) | (need_new ? 2 : 0);
     if (!ready) {
-        if (sk == RETRY) return;
+        if (sk == RETRY && !respec) return 0;
         flags |= 4;
     }
-    if (atomicCAS(c.state + k, sk, QUEUED) == sk) {
-        c.qflags[k] = flags;
+    if (atomicCAS(c.state + k, sk, QUEUED) != sk) return 0;
+    c.qflags[k] = flags | ((round < (1 << 22) ? round : (1 << 22)) << 8);
+    return 2;
+}
+
+__global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= c.m) return;
+    int prev;
+    if (flip_try(c, k, round, &prev) == 2) {
         const int qi = atomicAdd(&c.s->qcount[round & 1], 1);
         c.queue[round & 1][qi] = k;
     }
@@ -1455,6 +1481,224 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
 }
 
 
+// The same test run by one wavefront, for the small components most
+// searches meet: a 1024-slot LDS hash of visited cells, frontiers of up to
+// 256 cells, the frontier appends ordered by ballot prefix within the
+// wavefront, colour contacts and presence held in registers (uniform over
+// the wavefront), so no workgroup barrier is needed.  Returns 1 connected,
+// 0 disconnected, -1 when the search outgrows the hash or a frontier.
+constexpr int WB_HCAP = 1024;       // hash slots (power of two)
+constexpr int WB_HMAX = 768;        // insertions before overflow
+constexpr int WB_FCAP = 256;        // frontier capacity
+constexpr int WB_WAVES = 4;         // wavefronts (searches) per 256-thread workgroup
+
+struct WaveBfs {
+    unsigned long long hash[WB_HCAP];   // (cell << 8) | colour, ~0 empty
+    int fr[2][WB_FCAP];
+    uint8_t frc[2][WB_FCAP];
+};
+
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = (unsigned long long)__shfl_xor((long long)v, off);
+        v |= o;
+    }
+    return v;
+}
+
+// colours merged through recorded contacts; 1 all one group, 0 a group with
+// no frontier left, -1 undecided (uniform over the wavefront)
+__device__ __forceinline__ int colour_verdict(int nb, unsigned long long adj, uint32_t present) {
+    int comp[6];
+    for (int q = 0; q < nb; ++q) comp[q] = q;
+    bool changed = true;
+    while (changed) {
+        changed = false;
+        for (int a = 0; a < nb; ++a)
+            for (int b = 0; b < nb; ++b)
+                if (((adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
+                    const int m = comp[a] < comp[b] ? comp[a] : comp[b];
+                    comp[a] = m; comp[b] = m; changed = true;
+                }
+    }
+    bool all = true;
+    for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
+    if (all) return 1;
+    for (int q = 0; q < nb; ++q) {
+        bool has = false;
+        for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1u)) has = true;
+        if (!has) return 0;
+    }
+    return -1;
+}
+
+template <int MODE>
+__device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_t L, bool excl, int k) {
+    const Dims& d = c.d;
+    const int nface = c.dim3 ? 6 : 4;
+    const int lane = threadIdx.x & 63;
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    for (int h = lane; h < WB_HCAP; h += 64) S.hash[h] = ~0ull;
+    // seeds: the face neighbours of label L, one colour each in face order (a
+    // neighbour met twice, on grids of side 2, is one cell of two colours)
+    int f[6];
+    faces(d, cell, f);
+    int seed[6], nb = 0;
+    unsigned long long adj = 0ull;
+    for (int q = 0; q < nface; ++q) {
+        if (view_label_b<MODE>(f[q], k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
+        seed[nb++] = f[q];
+    }
+    if (nb <= 1) return 1;
+    int nfront = 0;
+    uint32_t present = 0u;
+    wave_sync();
+    for (int q = 0; q < nb; ++q) {
+        int dup = -1;
+        for (int p = 0; p < q; ++p) if (seed[p] == seed[q] && dup < 0) dup = p;
+        if (dup >= 0) {
+            adj |= (1ull << (dup * 8 + q)) | (1ull << (q * 8 + dup));
+            continue;
+        }
+        if (lane == 0) {
+            uint32_t h = (uint32_t(seed[q]) * 2654435761u) >> (32 - 10);
+            while (S.hash[h] != ~0ull) h = (h + 1) & (WB_HCAP - 1);
+            S.hash[h] = ((unsigned long long)(uint32_t)seed[q] << 8) | uint32_t(q);
+            S.fr[0][nfront] = seed[q];
+            S.frc[0][nfront] = uint8_t(q);
+        }
+        ++nfront;
+        present |= 1u << q;
+    }
+    wave_sync();
+    int cur = 0, inserted = nfront;
+    while (true) {
+        const int v = colour_verdict(nb, adj, present);
+        if (v >= 0) return v;
+        const int nxt = cur ^ 1;
+        int ncount = 0;
+        bool ovf = false;
+        unsigned long long myadj = 0ull;
+        uint32_t mypres = 0u;
+        for (int e0 = 0; e0 < nfront; e0 += 64) {
+            const int e = e0 + lane;
+            const bool act = e < nfront;
+            const int cc = act ? S.fr[cur][e] : 0;
+            const int col = act ? int(S.frc[cur][e]) : 0;
+            int g6[6];
+            faces(d, cc, g6);
+            // the neighbours' labels in the view: the candidate ids and labels
+            // of all faces are loaded before any is used, then the (rare)
+            // earlier candidates among them are resolved
+            int cid[6];
+            int32_t lab[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                cid[q] = -1;
+                lab[q] = -1;
+                if (act && q < nface) {
+                    cid[q] = c.cand_id[g6[q]];
+                    lab[q] = c.phi[g6[q]];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const int j = cid[q];
+                if (j >= 0 && j < k) {
+                    const int sj = ld_acq(c.state + j);
+                    if (sj == ACCEPT) lab[q] = c.Cnew[j];
+                    else if (MODE == VIEW_UPPER && sj != REJECT && c.Cnew[j] == L) lab[q] = L;
+                    else if (MODE == VIEW_LOWER && sj != REJECT && c.Cold[j] == L) lab[q] = -1;
+                }
+            }
+            for (int q = 0; q < nface; ++q) {
+                const int nbc = g6[q];
+                bool ins = false;
+                if (act && !(excl && nbc == cell) && lab[q] == L) {
+                    const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | uint32_t(col);
+                    uint32_t h = (uint32_t(nbc) * 2654435761u) >> (32 - 10);
+                    int probe = 0;
+                    for (; probe < WB_HCAP; ++probe) {
+                        const unsigned long long prev = atomicCAS(&S.hash[h], ~0ull, val);
+                        if (prev == ~0ull) { ins = true; break; }
+                        if (int(prev >> 8) == nbc) {
+                            const int oc = int(prev & 0xffull);
+                            if (oc != col) myadj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
+                            break;
+                        }
+                        h = (h + 1) & (WB_HCAP - 1);
+                    }
+                    if (probe == WB_HCAP) ovf = true;
+                }
+                const unsigned long long bal = __ballot(ins);
+                const int pos = ncount + __popcll(bal & lt);
+                if (ins) {
+                    if (pos < WB_FCAP) {
+                        S.fr[nxt][pos] = nbc;
+                        S.frc[nxt][pos] = uint8_t(col);
+                    }
+                    mypres |= 1u << col;
+                }
+                ncount += __popcll(bal);
+            }
+        }
+        inserted += ncount;
+        adj |= wave_or64(myadj);
+        present = uint32_t(wave_or64(mypres));
+        if (__ballot(ovf) != 0ull || ncount > WB_FCAP || inserted > WB_HMAX) return -1;
+        nfront = ncount;
+        cur = nxt;
+        wave_sync();
+    }
+}
+
+// One wavefront per queued candidate (WB_WAVES per workgroup).  Exact turns
+// and bounds searches as k_flip_bfs; a search that outgrows the wavefront's
+// hash queues the candidate (still QUEUED) for k_flip_bfs's workgroup
+// search.  Block 0 clears the counts that the next round fills.
+__global__ void __launch_bounds__(256) k_flip_bfs_wave(FlipCtx c, int round, int* __restrict__ ovq,
+                                                        int* __restrict__ ovcount) {
+    __shared__ WaveBfs S[WB_WAVES];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c.s->qcount[(round + 1) & 1] = 0;
+        ovcount[(round + 1) & 1] = 0;
+    }
+    const int n = c.s->qcount[round & 1];
+    for (int qi = blockIdx.x * WB_WAVES + w; qi < n; qi += gridDim.x * WB_WAVES) {
+        const int k = c.queue[round & 1][qi];
+        const int flags = c.qflags[k];
+        const int cell = c.Ccell[k];
+        int verdict;    // ACCEPT, REJECT, RETRY, or -1: to the workgroup search
+        if (!(flags & 4)) {
+            int ok = 1;
+            if (flags & 1) ok = bfs_connected_wave<VIEW_EXACT>(S[w], c, cell, c.Cold[k], true, k);
+            if (ok == 1 && (flags & 2)) ok = bfs_connected_wave<VIEW_EXACT>(S[w], c, cell, c.Cnew[k], false, k);
+            verdict = ok < 0 ? -1 : (ok ? ACCEPT : REJECT);
+        } else {
+            bool fail = false, open = false, big = false;
+            for (int t = 0; t < 2 && !fail; ++t) {
+                if (!(flags & (1 << t))) continue;
+                const int32_t L = t == 0 ? c.Cold[k] : c.Cnew[k];
+                const bool excl = t == 0;
+                const int up = bfs_connected_wave<VIEW_UPPER>(S[w], c, cell, L, excl, k);
+                if (up == 0) { fail = true; break; }
+                if (up < 0) { big = true; continue; }
+                const int lo = bfs_connected_wave<VIEW_LOWER>(S[w], c, cell, L, excl, k);
+                if (lo < 0) big = true;
+                else if (lo != 1) open = true;
+            }
+            verdict = fail ? REJECT : (big ? -1 : (open ? RETRY : ACCEPT));
+        }
+        if (lane == 0) {
+            if (verdict == ACCEPT || verdict == REJECT) decide(c, k, verdict);
+            else if (verdict == RETRY) st_rel(c.state + k, RETRY);
+            else ovq[atomicAdd(&ovcount[round & 1], 1)] = k;
+        }
+    }
+}
+
 // Large-component fallback: the same multi-source test with the visited set
 // in device memory and frontiers of grid capacity.  A cell's mark packs the
 // transaction tag and the colour ((tag << 3) | colour), so no clearing is
@@ -1577,18 +1821,19 @@ __device__ void flip_bfs_fallback(FlipCtx& c, const int* __restrict__ fbq, int* 
     if (threadIdx.x == 0) c.s->fbcount = 0;
 }
 
-// One workgroup per queued candidate; candidates whose search outgrows the
-// LDS visited set go to the fallback queue, which the last workgroup to
-// finish runs with the device-memory search before the kernel ends.
-__global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, int round, int* __restrict__ fbq,
+// One workgroup per queued candidate (the candidates whose wavefront search
+// outgrew its hash); candidates whose search outgrows this LDS visited set
+// go to the fallback queue, which the last workgroup to finish runs with the
+// device-memory search before the kernel ends.
+__global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, const int* __restrict__ queue,
+                                                   const int* __restrict__ qcount, int* __restrict__ fbq,
                                                    int* __restrict__ mark, int* __restrict__ fr0,
                                                    int* __restrict__ fr1, int* __restrict__ tag_ctr) {
     __shared__ BfsShared S;
     __shared__ int last;
-    if (blockIdx.x == 0 && threadIdx.x == 0) c.s->qcount[(round + 1) & 1] = 0;
-    const int n = c.s->qcount[round & 1];
+    const int n = *qcount;
     for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
-        const int k = c.queue[round & 1][qi];
+        const int k = queue[qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
         if (!(flags & 4)) {
