@@ -8,9 +8,17 @@ The headline `value` is the FP64 GPU path: sum over the K timed steps of the
 band size |A_s| divided by the wall time of those steps, with the state
 already resident in HBM.  The FP32 path is timed the same way and reported
 beside it, with the curvature error against FP64 on identical labels and the
-label disagreement after the run.  The CPU baseline times the reference step
+label disagreement after the run.  `full_run` times the whole configured
+run (--full-steps, 200 by default) per precision from the Voronoi state,
+step by step, with the band size range and, at its end, the FP32 label
+disagreement and curvature error.  The CPU baseline times the reference step
 (oracle/_ref, built from the reference sources) on this host's cores for a
-bounded sample of the same workload.
+bounded sample of the same workload: one untimed warm-up step, then
+--cpu-steps timed steps (3 by default), recording the CPU model and the
+OpenMP thread count.  `roofline` is the geometry kernel's executed FP64 flops
+per launch over its HIP-event launch time against the FP64 vector peak (its
+dense-formulation rate under `effective`), with the HBM bytes per launch
+from the committed rocprofv3 counter summary.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 
