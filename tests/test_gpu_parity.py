@@ -80,6 +80,15 @@ def test_grain_growth_sigma2_3d_strict32():
     lockstep(o, G, 24, check_every=3)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+@pytest.mark.parametrize("sigma", [1.0, 2.0, 2.6667])
+def test_fast_modes_other_radii_3d(mode, sigma):
+    # 3-D row-table geometry at window radii 6, 9 and 12 (sigma 1.5 is radius 8)
+    phi, g = O.voronoi(32, 32, 32, 40, 11)
+    o, G = pair((32, 32, 32), _abi.isotropic(1.0, 1.0), 0.0, sigma, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 24, check_every=4)
+
+
 @pytest.mark.parametrize("mode", ["strict32", "fp32", "fp64"])
 def test_broad_kernel_geometry_3d(mode):
     # sigma 5 cells: 3-D geometry radius 20 > 18, per-label separable
