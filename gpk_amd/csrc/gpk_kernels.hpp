@@ -8,6 +8,8 @@
 // every kernel is deterministic and reproduces the CPU restatement's
 // operation order.
 #pragma once
+#include <type_traits>
+
 #include "gpk_device.hpp"
 #include "gpk_cell.hpp"
 
@@ -165,28 +167,39 @@ k_geometry(Dims d, GeomParams gp, gpk_model_desc model, int nband,
 // One workgroup per TZ x TY x TX tile.  The tile's interface cells are listed
 // and their resident labels deduplicated in an LDS hash.  For a batch of up to
 // MAXRES residents, every staged row (tile + RG halo, SX <= 64 cells) is read
-// by one wavefront and turned into one 64-bit "same label" mask per resident
-// with a single compare + ballot (bit = staged column).  Each interface cell
+// by one wavefront (or half of one when SX <= 32) and turned into one "same
+// label" mask per resident with a single compare + ballot (bit = staged
+// column; 32-bit words when SX <= 32, else 64-bit).  Each interface cell
 // then takes its window row mask as (row mask >> lx) and reads the row sums
 // from the chunk tables; y and z follow with FMA.  Same operation order as
 // k_geometry<R,DIM,false>.  Tiles are assigned XCD-major (tile t =
 // (id % 8) * per_xcd + id / 8) so neighbouring tiles share one XCD's L2.
 // ---------------------------------------------------------------------------
-template <int DIM> struct TileShape;
-template <> struct TileShape<3> { static constexpr int TZ = 8, TY = 8, TX = 32; };
-template <> struct TileShape<2> { static constexpr int TZ = 1, TY = 64, TX = 32; };
+// 3-D tiles up to radius 8 are 16 cells wide in x, so that a staged row
+// (16 + 2 * 8 cells) is one 32-bit mask and one wavefront load covers two
+// staged rows; wider radii keep 32-cell-wide tiles with 64-bit masks.
+template <int DIM, int RG> struct TileShape;
+template <int RG> struct TileShape<3, RG> {
+    static constexpr int TZ = RG <= 8 ? 16 : 8, TY = 8, TX = RG <= 8 ? 16 : 32;
+};
+template <int RG> struct TileShape<2, RG> { static constexpr int TZ = 1, TY = 64, TX = 32; };
 
-constexpr int GEO_HCAP = 256;    // resident hash slots per tile
+constexpr int GEO_HCAP = 256;    // resident hash slots per tile (resident indices fit 8 bits)
 
 template <class R, int DIM, int RG>
 struct TiledLds {
-    // residents per mask batch: 3 in FP64 and 4 in FP32 keep a 256-thread
-    // workgroup's LDS within a quarter of the CU's 160 KB at radius 8
-    static constexpr int MAXRES = sizeof(R) == 8 ? 3 : 4;
-    static constexpr int MAXB = (GEO_HCAP + MAXRES - 1) / MAXRES;   // batches per tile
-    static constexpr int TZ = TileShape<DIM>::TZ, TY = TileShape<DIM>::TY, TX = TileShape<DIM>::TX;
+    static constexpr int TZ = TileShape<DIM, RG>::TZ, TY = TileShape<DIM, RG>::TY, TX = TileShape<DIM, RG>::TX;
     static constexpr int HZ = DIM == 3 ? RG : 0;
     static constexpr int SZ = TZ + 2 * HZ, SY = TY + 2 * RG, SX = TX + 2 * RG;
+    // row mask word: 32 bits when a staged row fits them (then one wavefront
+    // load stages RPL = 2 rows, one per half-wavefront)
+    static constexpr bool NARROW = SX <= 32;
+    using M = typename std::conditional<NARROW, uint32_t, unsigned long long>::type;
+    static constexpr int RPL = NARROW ? 2 : 1;
+    // residents per mask batch: as many as keep a 256-thread workgroup's LDS
+    // within a quarter of the CU's 160 KB at radius 8 (four workgroups per CU)
+    static constexpr int MAXRES = NARROW ? (sizeof(R) == 8 ? 5 : 7) : (sizeof(R) == 8 ? 3 : 4);
+    static constexpr int MAXB = (GEO_HCAP + MAXRES - 1) / MAXRES;   // batches per tile
     static constexpr int NROWS = SZ * SY;
     static constexpr int W = 2 * RG + 1;
     static constexpr int NCH = (W + 8) / 9;
@@ -235,14 +248,15 @@ __device__ __forceinline__ void tiled_cell(
         R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
 #pragma unroll 8
         for (int jy = 0; jy < S::W; ++jy) {
-            const unsigned long long m = rowmask(jz, jy);
+            const auto m = rowmask(jz, jy);
+            using MW = decltype(m + 0u);
             R R0, R1, R2;
             int base = 0;
 #pragma unroll
             for (int k = 0; k < S::NCH; ++k) {
                 const int start = k * S::C;
                 const int len = (S::W - start) < S::C ? (S::W - start) : S::C;
-                const int ix = int((m >> start) & ((1ull << len) - 1ull));
+                const int ix = int((MW(m) >> start) & ((MW(1) << len) - MW(1)));
                 R t0, t1, t2;
                 if constexpr (S::LS == 4) {
                     const float4 v = *reinterpret_cast<const float4*>(lut + base + ix * 4);
@@ -314,18 +328,21 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
                  int32_t* __restrict__ phi_comp, R* __restrict__ g_comp) {
     using S = TiledLds<R, DIM, RG>;
     constexpr int NT = S::TZ * S::TY * S::TX;
-    static_assert(NT <= 4096 && S::MAXRES <= 16, "tile index and batch slot must fit 16 bits");
+    using M = typename S::M;
+    static_assert(NT <= 4096 && S::MAXRES <= 8, "tile index, batch slot and pair flag must fit 16 bits");
     // rm holds the row masks of one resident batch; before the first batch the
     // same memory holds the tile's interface list and each listed cell's
     // resident index (2 + 2 bytes per cell)
-    constexpr int NRM = S::NROWS * S::MAXRES > NT / 2 ? S::NROWS * S::MAXRES : NT / 2;
-    __shared__ unsigned long long rm[NRM];
+    constexpr int NLC = NT * 4 / int(sizeof(M));
+    constexpr int NRM = S::NROWS * S::MAXRES > NLC ? S::NROWS * S::MAXRES : NLC;
+    __shared__ M rm[NRM];
     short* const list = reinterpret_cast<short*>(rm);
     unsigned short* const cres = reinterpret_cast<unsigned short*>(rm) + NT;
     __shared__ __align__(16) R lut[S::NLUT_LDS];
-    __shared__ uint16_t blist[NT];        // (slot in the batch << 12) | tile cell, grouped by batch
+    // (pair << 15) | (slot in the batch << 12) | tile cell, grouped by batch
+    __shared__ uint16_t blist[NT];
     __shared__ int hkey[GEO_HCAP];
-    __shared__ short hidx[GEO_HCAP];
+    __shared__ uint8_t hidx[GEO_HCAP];    // resident index of a hash slot (< GEO_HCAP)
     __shared__ int resl[GEO_HCAP];
     __shared__ int bcount[S::MAXB + 1], boff[S::MAXB + 1];
     __shared__ int zoff[S::SZ], yoff[S::SY];    // wrapped plane / row offsets of the staged rows
@@ -392,7 +409,7 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
     for (int h = threadIdx.x; h < GEO_HCAP; h += blockDim.x) {
         if (hkey[h] != -1) {
             const int k = atomicAdd(&nres, 1);
-            hidx[h] = short(k);
+            hidx[h] = uint8_t(k);
             resl[k] = hkey[h];
         }
     }
@@ -440,55 +457,70 @@ k_geometry_tiled(Dims d, GeomParams gp, gpk_model_desc model, int ntx, int nty, 
         const int b = r == 0xFFFFu ? S::MAXB : int(r) / S::MAXRES;
         const uint32_t t = r == 0xFFFFu ? 0u : r % S::MAXRES;
         const int pos = pr ? boff[b] + atomicAdd(&fcnt[b], 1) : boff[b] + bcount[b] - 1 - atomicAdd(&bcnt[b], 1);
-        blist[pos] = uint16_t((pr << 14) | (t << 12) | uint32_t(list[q]));
+        blist[pos] = uint16_t((pr << 15) | (t << 12) | uint32_t(list[q]));
     }
     const int nr = nres;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    const int xw = lane < S::SX ? wrapi(x0t + lane - RG, d.Nx) : 0;
-    constexpr int INFL = 4;               // staged rows in flight per wavefront
+    // staged column of this lane: with 32-bit masks each half-wavefront
+    // stages its own row (h = which one)
+    const int sl = S::NARROW ? (lane & 31) : lane;
+    const int h = S::NARROW ? (lane >> 5) : 0;
+    const int xw = sl < S::SX ? wrapi(x0t + sl - RG, d.Nx) : 0;
+    constexpr int INFL = 4;                  // wavefront loads in flight
+    constexpr int RPP = INFL * S::RPL;       // staged rows per wavefront pass
     for (int b0 = 0; b0 < nr; b0 += S::MAXRES) {
         const int nbat = nr - b0 < S::MAXRES ? nr - b0 : S::MAXRES;
         const int bb = b0 / S::MAXRES;
         __syncthreads();
-        // row masks: wave per staged row, one ballot per resident of the
-        // batch; the INFL x MAXRES ballots of a pass are gathered into lanes
-        // k * MAXRES + t (branch-free selects) and stored by one LDS write
-        static_assert(INFL * S::MAXRES <= 64, "one lane per ballot of a pass");
+        // row masks: each wavefront pass stages RPP consecutive rows (RPL per
+        // load), one ballot per load and resident of the batch; the mask
+        // words are gathered into lanes (k * MAXRES + t) * RPL + row-of-load
+        // by branch-free selects and stored by one LDS write per lane
+        static_assert(INFL * S::MAXRES * S::RPL <= 64, "one lane per mask word of a pass");
         int rv[S::MAXRES];
 #pragma unroll
         for (int t = 0; t < S::MAXRES; ++t) rv[t] = t < nbat ? resl[b0 + t] : -2;   // -2: no label
-        for (int r0 = wave; r0 < S::NROWS; r0 += INFL * nwave) {
+        for (int r0 = wave * RPP; r0 < S::NROWS; r0 += RPP * nwave) {
             int v[INFL];
 #pragma unroll
             for (int k = 0; k < INFL; ++k) {
-                const int r = r0 + k * nwave;
+                const int r = r0 + k * S::RPL + h;
                 v[k] = -1;
-                if (r < S::NROWS && lane < S::SX) v[k] = phi[zoff[r / S::SY] + yoff[r % S::SY] + xw];
+                if (r < S::NROWS && sl < S::SX) v[k] = phi[zoff[r / S::SY] + yoff[r % S::SY] + xw];
             }
-            int mlo = 0, mhi = 0;
+            int mw0 = 0, mw1 = 0;   // mask words (64-bit masks: low, high halves)
+            const int qk = lane / S::RPL;
+            const bool odd = (lane & 1) != 0;
 #pragma unroll
             for (int k = 0; k < INFL; ++k)
 #pragma unroll
                 for (int t = 0; t < S::MAXRES; ++t) {
                     const unsigned long long msk = __ballot(v[k] == rv[t]);
-                    const bool mine = lane == k * S::MAXRES + t;
-                    mlo = mine ? int(uint32_t(msk)) : mlo;
-                    mhi = mine ? int(uint32_t(msk >> 32)) : mhi;
+                    const bool mine = qk == k * S::MAXRES + t;
+                    if constexpr (S::NARROW) {
+                        const uint32_t w = odd ? uint32_t(msk >> 32) : uint32_t(msk);
+                        mw0 = mine ? int(w) : mw0;
+                    } else {
+                        mw0 = mine ? int(uint32_t(msk)) : mw0;
+                        mw1 = mine ? int(uint32_t(msk >> 32)) : mw1;
+                    }
                 }
-            const int kr = lane / S::MAXRES, tr = lane - kr * S::MAXRES;
-            const int r = r0 + kr * nwave;
-            if (lane < INFL * S::MAXRES && r < S::NROWS)
-                rm[r * S::MAXRES + tr] = (uint64_t(uint32_t(mhi)) << 32) | uint32_t(mlo);
+            const int kr = qk / S::MAXRES, tr = qk - kr * S::MAXRES;
+            const int r = r0 + kr * S::RPL + (odd && S::NARROW ? 1 : 0);
+            if (lane < INFL * S::MAXRES * S::RPL && r < S::NROWS) {
+                if constexpr (S::NARROW) rm[r * S::MAXRES + tr] = M(uint32_t(mw0));
+                else rm[r * S::MAXRES + tr] = (M(uint32_t(mw1)) << 32) | M(uint32_t(mw0));
+            }
         }
         __syncthreads();
         const int bend = boff[bb] + bcount[bb];
         for (int u = boff[bb] + threadIdx.x; u < bend; u += blockDim.x) {
             const uint32_t pk = blist[u];
             const int e = int(pk & 0xFFFu);
-            const int t = int((pk >> 12) & 3u);
-            const bool pr = (pk >> 14) & 1u;
+            const int t = int((pk >> 12) & 7u);
+            const bool pr = (pk >> 15) & 1u;
             const int lx = e % S::TX, ly = (e / S::TX) % S::TY, lz = e / (S::TX * S::TY);
-            const unsigned long long* rows = rm + (lz * S::SY + ly) * S::MAXRES + t;
+            const M* rows = rm + (lz * S::SY + ly) * S::MAXRES + t;
             tiled_cell<R, DIM, RG, LAW>(d, gp, model, x0t + lx, y0t + ly, z0t + lz, pr, resl[b0 + t], lut, w0, w1, w2,
                                    [&](int jz, int jy) {
                                        // the extra plane of a pair; clamped for the
