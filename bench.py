@@ -564,6 +564,8 @@ def main():
         # masked stencil is not a contraction and uses no matrix cores
         "roofline": {"bound": "valu", "compute": "FP64 vector ALU (VALU), no MFMA",
                      "kernel": "k_geometry_tiled<double,3,8> (separable row-mask tables)",
+                     "limiter": "LDS table reads (indexed LDS operations 70% of CU-cycles, bank "
+                                "conflicts 37%; profiles/pmc_geometry_fp64_256.md)",
                      "achieved": exec_rate, "peak": PEAK_TFLOPS["fp64"], "unit": "TFLOP/s",
                      "frac": exec_rate / PEAK_TFLOPS["fp64"], "traffic": traffic,
                      "traffic_source": traffic_src,
