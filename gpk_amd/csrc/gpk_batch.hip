@@ -34,6 +34,7 @@
 
 #include "gpk_cell.hpp"
 #include "gpk_host.hpp"
+#include "gpk_tiles.hpp"
 
 using namespace gpkd;
 using namespace gpkh;
@@ -91,8 +92,20 @@ struct BatchArgs {
     int* band;
     int* slot;
     R *nx, *ny, *nz, *kappa, *vraw, *vn, *gain;
+    uint8_t* pflag;       // remap proposal flag per band slot (tiled scoring)
     Prop* props;
 };
+
+// LDS of the tiled stages (FP32 / FP64 modes): geometry at radius 8
+// (sigma 1.5-2.0 cells), projection and remap scoring at radius 3; the
+// stages run one after another, so they share one block
+template <class R>
+union BatchTileSmem {
+    GeoTileSmem<R, 2, 8> geo;
+    ProjTileSmem<R, 2, 3> proj;
+    RemapTileSmem<2, 3> remap;
+};
+struct NoTileSmem {};
 
 __device__ __forceinline__ uint64_t orderable(float v) {
     const uint32_t b = __float_as_uint(v);
@@ -328,8 +341,8 @@ __device__ __forceinline__ void cube_masks(const Dims& d, const int32_t* __restr
     }
 }
 
-template <class R, int WG, bool kDirect>
-__device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
+template <class R, int WG, bool kDirect, class TS>
+__device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, TS& tl, int b) {
     const Inst I = A.inst[b];
     const Dims d{1, I.Ny, I.Nx, I.N};
     const int tid = threadIdx.x;
@@ -355,19 +368,52 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
 
     // band (src/gpk_geometry.cpp:42-82)
     const int nb = band_compact<WG>(sh, d, phi, mask, band, slot, true);
-    // geometry, competitor, constitutive velocity
-    for (int a = tid; a < nb; a += WG)
-        geometry_cell<R, 2, kDirect>(d, A.gp, A.model, a, band[a], phi, A.w0, A.w1, A.w2, A.tab, A.lutd, g,
-                                     c, T, static_cast<const R*>(nullptr), static_cast<const R*>(nullptr), bnx, bny, bnz, bk, vraw, phi_comp, g_comp);
+    // geometry, competitor, constitutive velocity: in FP32 / FP64 at radius 8
+    // the instance's tiles one after another out of LDS (the grid kernels'
+    // tile code, same arithmetic as the per-cell path), else cell by cell
+    bool tiled_geo = false;
+    if constexpr (!kDirect) {
+        if (A.gp.rg == 8) {
+            using S = TiledLds<R, 2, 8>;
+            const int ntx = (d.Nx + S::TX - 1) / S::TX, nty = (d.Ny + S::TY - 1) / S::TY;
+            geometry_tile_lut(tl.geo, A.tab);
+            for (int t = 0; t < ntx * nty; ++t)
+                geometry_tile<R, 2, 8, 0>(tl.geo, t, d, A.gp, A.model, ntx, nty, phi, mask, slot, A.w0, A.w1, A.w2,
+                                          g, c, T, static_cast<const R*>(nullptr), static_cast<const R*>(nullptr),
+                                          bnx, bny, bnz, bk, vraw, phi_comp, g_comp);
+            tiled_geo = true;
+        }
+    }
+    if (!tiled_geo)
+        for (int a = tid; a < nb; a += WG)
+            geometry_cell<R, 2, kDirect>(d, A.gp, A.model, a, band[a], phi, A.w0, A.w1, A.w2, A.tab, A.lutd, g,
+                                         c, T, static_cast<const R*>(nullptr), static_cast<const R*>(nullptr), bnx, bny, bnz, bk, vraw, phi_comp, g_comp);
     __syncthreads();
     // pair projection and the interface speed bound
     R vm = R(Eps<R>::vmax0);
-    for (int a = tid; a < nb; a += WG) {
-        const R v = project_value<R, 2>(d, A.proj_r, a, band[a], slot, mask, phi, phi_comp, A.projw, vraw);
-        vn[a] = v;
-        const R av = dfabs(v);
-        vm = vm < av ? av : vm;
+    bool tiled_proj = false;
+    if constexpr (!kDirect) {
+        if (A.proj_r == 3) {
+            using H = HaloGeom<2, 3>;
+            const int ntx = (d.Nx + H::TX - 1) / H::TX, nty = (d.Ny + H::TY - 1) / H::TY;
+            typename UBits<R>::T mb = 0;
+            for (int t = 0; t < ntx * nty; ++t) {
+                const auto m = project_tile<R, 2, 3>(tl.proj, t, d, ntx, nty, mask, phi, phi_comp, slot, A.projw,
+                                                     vraw, vn);
+                mb = mb < m ? m : mb;
+            }
+            const R av = bits_abs<R>(mb);
+            vm = vm < av ? av : vm;
+            tiled_proj = true;
+        }
     }
+    if (!tiled_proj)
+        for (int a = tid; a < nb; a += WG) {
+            const R v = project_value<R, 2>(d, A.proj_r, a, band[a], slot, mask, phi, phi_comp, A.projw, vraw);
+            vn[a] = v;
+            const R av = dfabs(v);
+            vm = vm < av ? av : vm;
+        }
     const R vmax = block_max<WG, R>(sh, vm);
     // dt (src/gpk_simulation.cpp:11-40), D = 0
     const R dx = R(A.dx_f);
@@ -493,8 +539,41 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, int b) {
         for (int l = tid; l < BATCH_LMAX; l += WG) sh.hist[l] = 0;
         __syncthreads();
         for (int i = tid; i < d.N; i += WG) atomicAdd(&sh.hist[phi[i]], 1);
-        // scores and ordered proposal list
-        for (int base = 0; base < nb2; base += WG) {
+        // scores (tiled in FP32 / FP64 at radius 3: gain and proposal flag per
+        // band slot, best label in phi_comp) and the proposal list in band order
+        bool tiled_remap = false;
+        if constexpr (!kDirect) {
+            if (A.remap_r == 3) {
+                using H = HaloGeom<2, 3>;
+                const int ntx = (d.Nx + H::TX - 1) / H::TX, nty = (d.Ny + H::TY - 1) / H::TY;
+                uint8_t* pflag = A.pflag + I.off;
+                const ScoreCache<R> nocache{nullptr, nullptr, nullptr, nullptr};
+                for (int t = 0; t < ntx * nty; ++t)
+                    remap_score_tile<R, 2, 3>(tl.remap, t, d, ntx, nty, mask, phi, slot, A.remapw, phi_comp, pflag,
+                                              gain, nocache);
+                __syncthreads();
+                for (int base = 0; base < nb2; base += WG) {
+                    const int a = base + tid;
+                    bool f = false;
+                    Prop pr{};
+                    if (a < nb2 && pflag[a]) {
+                        const int i = band[a];
+                        const int32_t p = phi[i], bl = phi_comp[i];
+                        const int32_t lo = p < bl ? p : bl, hi = p < bl ? bl : p;
+                        pr.hdr = (uint32_t(lo) << 11) | (uint32_t(hi) << 1) | (p == lo ? 0u : 1u);
+                        pr.cell = i;
+                        pr.gk = ~orderable(gain[a]);
+                        f = true;
+                    }
+                    int tot;
+                    const int pos = block_scan_flag<WG>(sh, f, tot);
+                    if (f) props[np + pos] = pr;
+                    np += tot;
+                }
+                tiled_remap = true;
+            }
+        }
+        for (int base = 0; base < nb2 && !tiled_remap; base += WG) {
             const int a = base + tid;
             bool f = false;
             Prop pr{};
@@ -680,13 +759,14 @@ __global__ void k_batch_reset(int* next) {
 template <class R, int WG, bool kDirect>
 __global__ void __launch_bounds__(WG) k_batch_step(BatchArgs<R> A) {
     __shared__ Shared<WG> sh;
+    __shared__ typename std::conditional<kDirect, NoTileSmem, BatchTileSmem<R>>::type tl;
     while (true) {
         if (threadIdx.x == 0) sh.inst = atomicAdd(A.next, 1);
         __syncthreads();
         const int q = sh.inst;
         __syncthreads();
         if (q >= A.n_inst) break;
-        step_instance<R, WG, kDirect>(A, sh, A.order[q]);
+        step_instance<R, WG, kDirect>(A, sh, tl, A.order[q]);
     }
 }
 
@@ -718,7 +798,7 @@ struct Batch final : BatchBase {
     RowLutDesc lutd{};
     DevBuf inst, order, next, st;
     DevBuf phi, alpha, g, c, T, mask, phi_comp, g_comp, band, slot;
-    DevBuf nx, ny, nz, kappa, vraw, vn, gain, props;
+    DevBuf nx, ny, nz, kappa, vraw, vn, gain, pflag, props;
     DevBuf wt0, wt1, wt2, tab, projw, remapw;
     int wg = 256, nwg = 0;
     double clock_hz = 1e8;
@@ -785,7 +865,7 @@ struct Batch final : BatchBase {
         phi.alloc(n * 4); alpha.alloc(n * rs); g.alloc(n * rs); c.alloc(n * rs); T.alloc(n * rs);
         mask.alloc(n); phi_comp.alloc(n * 4); g_comp.alloc(n * rs); band.alloc(n * 4); slot.alloc(n * 4);
         nx.alloc(n * rs); ny.alloc(n * rs); nz.alloc(n * rs); kappa.alloc(n * rs); vraw.alloc(n * rs);
-        vn.alloc(n * rs); gain.alloc(n * rs);
+        vn.alloc(n * rs); gain.alloc(n * rs); pflag.alloc(n);
         props.alloc(size_t(Stot) * sizeof(Prop));
         HIPCHK(hipMemset(phi.p, 0, n * 4));
         HIPCHK(hipMemset(phi_comp.p, 0, n * 4));
@@ -881,7 +961,7 @@ struct Batch final : BatchBase {
         A.T = T.as<R>(); A.mask = mask.as<uint8_t>(); A.phi_comp = phi_comp.as<int32_t>();
         A.g_comp = g_comp.as<R>(); A.band = band.as<int>(); A.slot = slot.as<int>();
         A.nx = nx.as<R>(); A.ny = ny.as<R>(); A.nz = nz.as<R>(); A.kappa = kappa.as<R>();
-        A.vraw = vraw.as<R>(); A.vn = vn.as<R>(); A.gain = gain.as<R>();
+        A.vraw = vraw.as<R>(); A.vn = vn.as<R>(); A.gain = gain.as<R>(); A.pflag = pflag.as<uint8_t>();
         A.props = props.as<Prop>();
         return A;
     }

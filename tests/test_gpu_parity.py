@@ -215,3 +215,14 @@ def test_geometry_tile_hash_overflow_path(mode):
     phi = rng.integers(0, 3000, size=(1, 96, 96)).astype(np.int32).reshape(96, 96)
     o, G = pair((1, 96, 96), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi})
     lockstep(o, G, 2)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+def test_geometry_tile_hash_overflow_path_3d(mode):
+    # the same in 3-D: a 16x8x16 tile of random labels holds about 1,700
+    # residents, so most of its cells take the row-from-the-grid path, next to
+    # residents that fit the hash and partial tiles at the 24-cell faces
+    rng = np.random.default_rng(5)
+    phi = rng.integers(0, 5000, size=(24, 24, 24)).astype(np.int32)
+    o, G = pair((24, 24, 24), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi})
+    lockstep(o, G, 2)
