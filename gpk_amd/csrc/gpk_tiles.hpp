@@ -94,7 +94,10 @@ __device__ __forceinline__ void tiled_cell(
     const int nzw = DIM == 3 ? S::W + (__ballot(pair) != 0ull ? 1 : 0) : 1;
     for (int jz = 0; jz < nzw; ++jz) {
         R S00 = 0, S01 = 0, S02 = 0, S10 = 0, S11 = 0, S20 = 0;
-#pragma unroll 8
+        // row loop unrolled fully in FP32 and by 8 in FP64 (measured: the
+        // FP64 build runs out of registers with the full unroll)
+        constexpr int UNR = sizeof(R) == 4 ? S::W : 8;
+#pragma unroll UNR
         for (int jy = 0; jy < S::W; ++jy) {
             const auto m = rowmask(jz, jy);
             using MW = decltype(m + 0u);
