@@ -549,8 +549,8 @@ __device__ void step_instance(const BatchArgs<R>& A, Shared<WG>& sh, TS& tl, int
                 uint8_t* pflag = A.pflag + I.off;
                 const ScoreCache<R> nocache{nullptr, nullptr, nullptr, nullptr};
                 for (int t = 0; t < ntx * nty; ++t)
-                    remap_score_tile<R, 2, 3>(tl.remap, t, d, ntx, nty, mask, phi, slot, A.remapw, phi_comp, pflag,
-                                              gain, nocache);
+                    remap_score_tile<R, 2, 3, false>(tl.remap, t, d, ntx, nty, mask, phi, slot, A.remapw, phi_comp,
+                                                     pflag, gain, nocache);
                 __syncthreads();
                 for (int base = 0; base < nb2; base += WG) {
                     const int a = base + tid;

@@ -548,9 +548,10 @@ __device__ __forceinline__ typename UBits<R>::T project_tile(
     for (int e = threadIdx.x; e < H::SY; e += blockDim.x) yoff[e] = wrapi(y0t + e - RP, d.Ny) * d.Nx;
     for (int e = threadIdx.x; e < H::SX; e += blockDim.x) xoff[e] = wrapi(x0t + e - RP, d.Nx);
     __syncthreads();
-    // key = ((low << 16 | high) << 1 | [resident is the high label]) for
-    // interface cells whose two labels differ, all ones otherwise (labels
-    // below 2^15, so the packed pair fits 31 bits and never equals all ones).
+    // key = (low << 16 | high) for interface cells whose two labels differ,
+    // all ones otherwise (labels below 2^15, so the packed pair fits 31 bits
+    // and never equals all ones); the velocity is staged with the cell's
+    // orientation in the pair, so a window cell is one key compare.
     // Four staged cells per thread are loaded at once: mask, labels and band
     // slot together, then the velocity of the interface ones.
     constexpr int INFL = 4;
@@ -583,7 +584,7 @@ __device__ __forceinline__ typename UBits<R>::T project_tile(
                 R vs = v[k];
                 if (mk[k] && p[k] != q[k]) {
                     const uint32_t lo = p[k] < q[k] ? p[k] : q[k], hi = p[k] < q[k] ? q[k] : p[k];
-                    kk = (((lo << 16) | hi) << 1) | (p[k] == lo ? 0u : 1u);
+                    kk = (lo << 16) | hi;
                     // staged with its orientation: w (-v) == -(w v) exactly
                     vs = p[k] == lo ? vs : -vs;
                 }
@@ -608,8 +609,10 @@ __device__ __forceinline__ typename UBits<R>::T project_tile(
         const uint32_t ki = key[c];
         R out = vv[c];
         if (ki != 0xFFFFFFFFu) {
-            const uint32_t pair = ki >> 1;
-            const R oi = (ki & 1u) ? R(-1) : R(1);
+            const uint32_t pair = ki;
+            // orientation of the centre: + when its label is the pair's low one
+            const int ic = lin(d, z0t + lz, y0t + ly, x0t + lx);
+            const R oi = phi[ic] < phi_comp[ic] ? R(1) : R(-1);
             double num = 0.0, den = 0.0;
             const int nzw = DIM == 3 ? W : 1;
             for (int jz = 0; jz < nzw; ++jz) {
@@ -628,7 +631,7 @@ __device__ __forceinline__ typename UBits<R>::T project_tile(
 #pragma unroll
                     for (int jx = 0; jx < W; ++jx) {
                         const int dxx = jx - RP;
-                        const bool hit = (kj[jx] >> 1) == pair;
+                        const bool hit = kj[jx] == pair;
                         const double w = wr[dxx * dxx];
                         const double t = w * double(vj[jx]);
                         num = add_if(hit, t, num);
@@ -723,13 +726,18 @@ struct RemapTileSmem {
     using H = HaloGeom<DIM, RS>;
     uint16_t lab[H::NS];
     short list[H::NT];
+    short order[H::NT];      // the listed cells by candidate count
+    uint8_t ncl[H::NT];      // candidate count of each listed cell
     int zoff[H::SZ], yoff[H::SY], xoff[H::SX];   // wrapped offsets of the staged box
     int nlist;
+    int ncnt[8];             // listed cells per candidate count, then their next slot
 };
 
 // remap scores of every interface cell of tile `tile` (labels < 2^16), run by
-// the whole workgroup
-template <class R, int DIM, int RS>
+// the whole workgroup; BYCOUNT: the cells in order of their candidate count
+// (pays where many workgroups share a CU; a workgroup that walks an
+// instance's tiles alone waits on the extra barriers instead)
+template <class R, int DIM, int RS, bool BYCOUNT = true>
 __device__ __forceinline__ void remap_score_tile(
     RemapTileSmem<DIM, RS>& sm, int tile, const Dims& d, int ntx, int nty, const uint8_t* __restrict__ mask,
     const int32_t* __restrict__ phi, const int* __restrict__ slot, const double* __restrict__ wtab_g,
@@ -778,13 +786,12 @@ __device__ __forceinline__ void remap_score_tile(
     }
     __syncthreads();
     const int nface = DIM == 3 ? 6 : 4;
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-        const int e = list[q];
+    // candidates of a listed cell: its distinct unlike face labels, in face order
+    auto candidates = [&](int e, int32_t& p, int32_t (&cand)[6]) {
         const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
         const int c = ((lz + H::HZ) * H::SY + (ly + RS)) * H::SX + (lx + RS);
-        const int32_t p = lab[c];
+        p = lab[c];
         const int fo[6] = {-1, 1, -H::SX, H::SX, -H::SX * H::SY, H::SX * H::SY};
-        int32_t cand[6];
         int nc = 0;
         for (int k = 0; k < nface; ++k) {
             const int32_t v = lab[c + fo[k]];
@@ -793,6 +800,34 @@ __device__ __forceinline__ void remap_score_tile(
             for (int t = 0; t < nc; ++t) known |= cand[t] == v;
             if (!known) cand[nc++] = v;
         }
+        return nc;
+    };
+    // the listed cells in order of their candidate count (counting sort), so
+    // that the lanes of a wavefront walk the same number of accumulators
+    // (a wavefront walks its largest count; most cells have one candidate)
+    if constexpr (BYCOUNT) {
+        if (threadIdx.x < 8) sm.ncnt[threadIdx.x] = 0;
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += blockDim.x) {
+            int32_t p, cand[6];
+            const int nc = candidates(list[q], p, cand);
+            sm.ncl[q] = uint8_t(nc);
+            atomicAdd(&sm.ncnt[nc], 1);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int o = 0;
+            for (int k = 0; k < 8; ++k) { const int c = sm.ncnt[k]; sm.ncnt[k] = o; o += c; }
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < n; q += blockDim.x) sm.order[atomicAdd(&sm.ncnt[sm.ncl[q]], 1)] = list[q];
+        __syncthreads();
+    }
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        const int e = BYCOUNT ? sm.order[q] : list[q];
+        const int lx = e % H::TX, ly = (e / H::TX) % H::TY, lz = e / (H::TX * H::TY);
+        int32_t p, cand[6];
+        const int nc = candidates(e, p, cand);
         // acc[0] = resident score, acc[1 + t] = support of candidate t; the
         // number of accumulators walked is the wavefront's largest nc + 1
         int32_t L[7];

@@ -199,3 +199,26 @@ def test_batch_full_scale_lamellar_fp32_guard_band():
         assert t[b] == o.time, (b, t[b], o.time)
     print("accepted", st["accepted"], "exchanges", st["exchanges"], "guarded", st["guarded"], flush=True)
     B.close()
+
+
+@pytest.mark.parametrize("mode", ["fp32", "fp64"])
+@pytest.mark.parametrize("sigma", [1.2, 2.0])
+def test_batch_other_geometry_radii(mode, sigma):
+    # sigma 1.2 (geometry radius 7) and 2.0 (radius 9) take the cell-by-cell
+    # geometry next to the tiled projection and remap scoring (radius 3)
+    shapes, phis, params = lamellar_sweep(6, seed=11, side_range=(40, 80), width_range=(4, 10),
+                                          ratio_range=(2.0, 5.0))
+    steps = 25
+    B = make_batch(shapes, phis, mode, sigma)
+    print("batch", mode, sigma, params, flush=True)
+    B.step(steps)
+    t, ms, nb = B.instances()
+    got = {f: B.get(f) for f in ("phi", "alpha", "g", "mask", "phi_comp")}
+    assert B.stats()["accepted"] > 0
+    for b, (shape, phi) in enumerate(zip(shapes, phis)):
+        o = oracle_run(shape, phi, mode, steps, sigma)
+        for f in ("phi", "alpha", "g", "mask", "phi_comp"):
+            ref = o.get(f)
+            assert np.array_equal(got[f][b], ref), (b, f, int(np.count_nonzero(got[f][b] != ref)))
+        assert t[b] == o.time, (b, t[b], o.time)
+    B.close()
