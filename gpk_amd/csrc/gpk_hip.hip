@@ -269,7 +269,7 @@ struct Engine final : EngineBase {
     RowLutDesc lutd{};      // row tables of the separable modes
 
     // ---- device state ----
-    DevBuf phi, mask, phi_comp, slot, cand_id, pid, mark;
+    DevBuf phi, mask, phi_comp, slot, pid, mark;
     DevBuf alpha, c, T, g, rho, g_comp, work1, work2, work3, work4;
     DevBuf bandA, bandB, nx, ny, nz, kappa, vraw, vn;
     DevBuf cflag, anew_s, sel;
@@ -355,7 +355,7 @@ struct Engine final : EngineBase {
 
     size_t device_bytes() const override {
         size_t t = 0;
-        const DevBuf* all[] = {&phi, &mask, &phi_comp, &slot, &cand_id, &pid, &mark, &alpha, &c, &T,
+        const DevBuf* all[] = {&phi, &mask, &phi_comp, &slot, &pid, &mark, &alpha, &c, &T,
                                &g, &rho, &g_comp, &work1, &work2, &work3, &work4, &bandA, &bandB,
                                &nx, &ny, &nz, &kappa, &vraw, &vn, &cflag, &anew_s, &sel, &Ccell,
                                &Cold, &Cnew, &Cg, &Canew, &state, &pos_old, &pos_new, &qflags,
@@ -384,7 +384,7 @@ struct Engine final : EngineBase {
     void allocate() {
         const size_t n = size_t(N), rs = sizeof(R);
         phi.alloc(n * 4); mask.alloc(n); phi_comp.alloc(n * 4); slot.alloc(n * 4);
-        cand_id.alloc(n * 4); pid.alloc(n * 4); mark.alloc(n * 4);
+        pid.alloc(n * 4); mark.alloc(n * 4);
         alpha.alloc(n * rs); c.alloc(n * rs); T.alloc(n * rs); g.alloc(n * rs); rho.alloc(n * rs);
         E_el.alloc(n * rs);
         rs_gain.alloc(n * rs); rs_best.alloc(n * 4); rs_valid.alloc(n); rs_dirty.alloc(n);
@@ -460,7 +460,6 @@ struct Engine final : EngineBase {
         fill<int32_t>(phi, N, 0);
         fill<uint8_t>(mask, N, 0);
         fill<int32_t>(phi_comp, N, 0);
-        fill<int32_t>(cand_id, N, -1);
         fill<int32_t>(pid, N, -1);
         fill<int32_t>(mark, N, 0);
         fill<R>(alpha, N, R(1));
@@ -1163,7 +1162,7 @@ struct Engine final : EngineBase {
         k_cand_init<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, sel.as<int>(), bandA.as<int>(), phi.as<int32_t>(), phi_comp.as<int32_t>(), g_comp.as<R>(),
             anew_s.as<R>(), Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(),
-            state.as<int>(), cand_id.as<int>(), network ? Pk.as<unsigned long long>() : nullptr);
+            state.as<int>(), network ? Pk.as<unsigned long long>() : nullptr);
         if (network) {
             // (label << 32 | k) keys are generated in k order, so a stable sort
             // on the label bits alone orders them by (label, k)
@@ -1179,7 +1178,7 @@ struct Engine final : EngineBase {
             FlipCtx fc;
             fc.d = dims; fc.dim3 = dim3; fc.m = m;
             fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
-            fc.state = state.as<int>(); fc.cand_id = cand_id.as<int>(); fc.phi = phi.as<int32_t>();
+            fc.state = state.as<int>(); fc.phi = phi.as<int32_t>();
             fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>();
             fc.pos_new = pos_new.as<int>(); fc.count = count.as<int>();
             fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
@@ -1205,8 +1204,13 @@ struct Engine final : EngineBase {
                 }
                 pull_scalars();
                 if (hs->undecided <= 0) break;
-                if (round > 4 * (m + 8))
+                if (round > 4 * (m + 8)) {
+                    // labels back from the candidate words before the error
+                    k_flip_restore<<<nblk(m, 256), 256, 0, stream>>>(m, Ccell.as<int>(), Cold.as<int>(),
+                                                                      phi.as<int32_t>());
+                    HIPCHK(hipStreamSynchronize(stream));
                     throw GpkError(GPK_E_HIP, "flip transaction did not converge");
+                }
                 batch = 4;
             }
             flip_batch = round > flip_batch ? round : std::max(4, flip_batch - 2);
@@ -1214,7 +1218,7 @@ struct Engine final : EngineBase {
         mask_fresh = false;
         k_flip_apply<R><<<nblk(m, 256), 256, 0, stream>>>(
             m, Ccell.as<int>(), Cold.as<int>(), Cnew.as<int>(), Cg.as<R>(), Canew.as<R>(), state.as<int>(),
-            phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), cand_id.as<int>(), ds(),
+            phi.as<int32_t>(), g.as<R>(), alpha.as<R>(), count.as<int>(), ds(),
             network ? 0 : 1, rho.as<R>(), R(model.rho_rex), sweep_rho());
         mark_dirty(m, 1, Ccell.as<int>(), nullptr, network ? state.as<int>() : nullptr);
         return m;
@@ -1472,7 +1476,7 @@ struct Engine final : EngineBase {
         FlipCtx fc;
         fc.d = dims; fc.dim3 = dim3; fc.m = 1;
         fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
-        fc.state = state.as<int>(); fc.cand_id = cand_id.as<int>(); fc.phi = phi.as<int32_t>();
+        fc.state = state.as<int>(); fc.phi = phi.as<int32_t>();
         fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>(); fc.pos_new = pos_new.as<int>();
         fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
         fc.qflags = qflags.as<int>(); fc.s = ds();

@@ -383,14 +383,18 @@ __global__ void k_noise_amp(const Scalars* s, float dx_f, float M0, float kB, fl
     *out = scale * sqrtf(2.f * M0 * kB * inv_dt / m);
 }
 
-// candidate table: cell, old/new label, new orientation, alpha_new
+// candidate table: cell, old/new label, new orientation, alpha_new.  With
+// ordered transactions (Pk given) each candidate's cell holds ~k in phi until
+// k_flip_apply writes its label back: labels lie in [0, 2^26), so a negative
+// label word names the candidate, and a neighbour's view costs one load of
+// phi plus, for candidate cells only, loads from the candidate table.
 template <class R>
 __global__ void k_cand_init(int m, const int* __restrict__ sel, const int* __restrict__ band,
-                            const int32_t* __restrict__ phi, const int32_t* __restrict__ phi_comp,
+                            int32_t* __restrict__ phi, const int32_t* __restrict__ phi_comp,
                             const R* __restrict__ g_comp, const R* __restrict__ anew_s,
                             int* __restrict__ Ccell, int* __restrict__ Cold, int* __restrict__ Cnew,
                             R* __restrict__ Cg, R* __restrict__ Canew, int* __restrict__ state,
-                            int* __restrict__ cand_id, unsigned long long* __restrict__ Pk) {
+                            unsigned long long* __restrict__ Pk) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= m) return;
     const int sa = sel[k];
@@ -399,8 +403,8 @@ __global__ void k_cand_init(int m, const int* __restrict__ sel, const int* __res
     Ccell[k] = cell; Cold[k] = ol; Cnew[k] = nl;
     Cg[k] = g_comp[cell]; Canew[k] = anew_s[sa];
     state[k] = UNDEC;
-    cand_id[cell] = k;
     if (Pk) {
+        phi[cell] = ~k;
         Pk[2 * k] = ((unsigned long long)(uint32_t)ol << 32) | (uint32_t)k;
         Pk[2 * k + 1] = ((unsigned long long)(uint32_t)nl << 32) | (uint32_t)k;
     }
@@ -417,13 +421,15 @@ __global__ void k_cand_pos(int n2, const unsigned long long* __restrict__ Ps,
     if (lab == Cold[k]) pos_old[k] = p; else pos_new[k] = p;
 }
 
-// label of `cell` as seen by transaction k (earlier accepted flips applied)
-__device__ __forceinline__ int32_t view_label(int cell, int k, const int32_t* phi,
-                                              const int* cand_id, const int* state,
-                                              const int* Cnew) {
-    const int j = cand_id[cell];
-    if (j >= 0 && j < k && ld_acq(state + j) == ACCEPT) return Cnew[j];
-    return phi[cell];
+// label of `cell` as seen by transaction k (earlier accepted flips applied);
+// phi holds ~j at the cell of candidate j while the transactions run
+__device__ __forceinline__ int32_t view_label(int cell, int k, const int32_t* phi, const int* state,
+                                              const int* Cold, const int* Cnew) {
+    const int32_t v = phi[cell];
+    if (v >= 0) return v;
+    const int j = ~v;
+    if (j < k && ld_acq(state + j) == ACCEPT) return Cnew[j];
+    return Cold[j];
 }
 
 // Views of label L bounding every outcome of the earlier transactions not
@@ -437,18 +443,61 @@ __device__ __forceinline__ int32_t view_label(int cell, int k, const int32_t* ph
 // connected at time k.  A transaction decided while a search runs moves a
 // cell from its bound value to its exact one, which keeps both properties.
 enum : int { VIEW_EXACT = 0, VIEW_UPPER = 1, VIEW_LOWER = 2 };
-template <int MODE>
-__device__ __forceinline__ int32_t view_label_b(int cell, int k, int32_t L, const int32_t* phi,
-                                                const int* cand_id, const int* state, const int* Cold,
-                                                const int* Cnew) {
-    const int j = cand_id[cell];
-    if (j >= 0 && j < k) {
+__device__ __forceinline__ int32_t view_label_b(int mode, int cell, int k, int32_t L, const int32_t* phi,
+                                                const int* state, const int* Cold, const int* Cnew) {
+    const int32_t v = phi[cell];
+    if (v >= 0) return v;
+    const int j = ~v;
+    if (j < k) {
         const int sj = ld_acq(state + j);
         if (sj == ACCEPT) return Cnew[j];
-        if (MODE == VIEW_UPPER && sj != REJECT && Cnew[j] == L) return L;
-        if (MODE == VIEW_LOWER && sj != REJECT && Cold[j] == L) return -1;
+        if (mode == VIEW_UPPER && sj != REJECT && Cnew[j] == L) return L;
+        if (mode == VIEW_LOWER && sj != REJECT && Cold[j] == L) return -1;
     }
-    return phi[cell];
+    return Cold[j];
+}
+
+// The tests of one queued transaction, written as one loop around a single
+// search call so that the search is inlined once (mode is a run-time view).
+// Exact turns test the old label (centre excluded), then the new label,
+// stopping at the first not connected.  Bounds turns test each label in the
+// upper view and, when that connects, in the lower view: one test
+// disconnected in the upper view refuses, all connected in the lower view
+// accept, anything else waits (RETRY).  A search that outgrew its visited set
+// (-1) makes the verdict -1 (searched again by a larger search), or, with
+// overflow_waits, leaves a bounds turn waiting.  search(mode, t) tests the
+// old label (t == 0) or the new label (t == 1).
+template <class F>
+__device__ __forceinline__ int flip_verdict(int flags, bool overflow_waits, F&& search) {
+    const bool bounds = (flags & 4) != 0;
+    bool fail = false, open = false, big = false;
+    int ok = 1, t = 0, phase = 0;
+    while (t < 2) {
+        if (!(flags & (1 << t))) { ++t; continue; }
+        const int mode = !bounds ? VIEW_EXACT : (phase == 0 ? VIEW_UPPER : VIEW_LOWER);
+        const int r = search(mode, t);
+        if (!bounds) {
+            ok = r;
+            if (ok != 1) break;
+            ++t;
+        } else if (phase == 0) {
+            if (r == 0) { fail = true; break; }
+            if (r < 0) {
+                if (overflow_waits) open = true;
+                else big = true;
+                ++t;
+            } else {
+                phase = 1;
+            }
+        } else {
+            if (r < 0 && !overflow_waits) big = true;
+            else if (r != 1) open = true;
+            phase = 0;
+            ++t;
+        }
+    }
+    if (!bounds) return ok < 0 ? -1 : (ok ? ACCEPT : REJECT);
+    return fail ? REJECT : (big ? -1 : (open ? RETRY : ACCEPT));
 }
 
 struct FlipCtx {
@@ -457,7 +506,6 @@ struct FlipCtx {
     int m;
     const int* Ccell; const int* Cold; const int* Cnew;
     int* state;
-    const int* cand_id;
     const int32_t* phi;
     const unsigned long long* Ps;
     const int* pos_old; const int* pos_new;
@@ -548,15 +596,19 @@ __device__ int flip_try(FlipCtx& c, int k, int round, int* prev) {
                 int x = x0 + ddx; if (x < 0) x += d.Nx; else if (x >= d.Nx) x -= d.Nx;
                 const int nb = lin(d, z, y, x);
                 if (nb == cell && dz == 0 && dy == 0 && ddx == 0) continue;
-                const int j = c.cand_id[nb];
                 int32_t lab = c.phi[nb];
-                if (j >= 0 && j < k) {
-                    const int jo = c.Cold[j], jn = c.Cnew[j];
-                    const int sj = ld_acq(c.state + j);
-                    if (jo == ol || jo == nl || jn == ol || jn == nl) {
-                        if (undecided(sj)) return 0;   // not ready
+                if (lab < 0) {
+                    const int j = ~lab;
+                    const int jo = c.Cold[j];
+                    lab = jo;
+                    if (j < k) {
+                        const int jn = c.Cnew[j];
+                        const int sj = ld_acq(c.state + j);
+                        if (jo == ol || jo == nl || jn == ol || jn == nl) {
+                            if (undecided(sj)) return 0;   // not ready
+                        }
+                        if (sj == ACCEPT) lab = jn;
                     }
-                    if (sj == ACCEPT) lab = jn;
                 }
                 if (lab == ol) occ_o |= 1u << t;
                 if (lab == nl) occ_n |= 1u << t;
@@ -615,6 +667,74 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
     }
 }
 
+// View labels (view_label_b) of the six face neighbours g6 of a frontier
+// cell for transaction k: the label words of all faces are loaded before any
+// is used, then the table entries of the candidates among them (label word
+// ~j; Cold and Cnew do not change while the rounds run) are loaded together
+// and resolved.  Faces past nface and inactive lanes read 0x7fffffff.
+__device__ __forceinline__ void face_view_labels(const FlipCtx& c, int mode, int k, int32_t L, bool act,
+                                                 int nface, const int g6[6], int32_t lab[6]) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        lab[q] = 0x7fffffff;
+        if (act && q < nface) lab[q] = c.phi[g6[q]];
+    }
+    int sj[6], cn[6], co[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        const int j = ~lab[q];
+        sj[q] = REJECT; cn[q] = -1; co[q] = -1;
+        if (lab[q] < 0) {
+            co[q] = c.Cold[j];
+            if (j < k) {
+                sj[q] = ld_acq(c.state + j);
+                cn[q] = c.Cnew[j];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        if (lab[q] < 0) {
+            int32_t r = co[q];
+            if (sj[q] == ACCEPT) r = cn[q];
+            else if (mode == VIEW_UPPER && sj[q] != REJECT && cn[q] == L) r = L;
+            else if (mode == VIEW_LOWER && sj[q] != REJECT && co[q] == L) r = -1;
+            lab[q] = r;
+        }
+    }
+}
+
+// Colour verdict of a multi-source search: 1 when the recorded contacts join
+// all nb colours into one group, 0 when some group has no frontier cell left
+// (it is closed: disconnected), -1 undecided.  Contact bit a*8+b joins
+// colours a and b; the groups are the transitive closure of the contacts,
+// found by three squarings of the 6x6 reachability rows (chains of up to
+// eight contacts; six colours need five).  Uniform values, no indexing.
+__device__ __forceinline__ int colour_verdict(int nb, unsigned long long adj, uint32_t present) {
+    const uint32_t all = (1u << nb) - 1u;
+    uint32_t row[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) row[q] = (uint32_t(adj >> (q * 8)) & 0x3fu) | (1u << q);
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        uint32_t nr[6];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            uint32_t a = row[q];
+#pragma unroll
+            for (int p = 0; p < 6; ++p) a |= ((row[q] >> p) & 1u) ? row[p] : 0u;
+            nr[q] = a;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q) row[q] = nr[q];
+    }
+    if ((row[0] & all) == all) return 1;
+    bool closed = false;
+#pragma unroll
+    for (int q = 0; q < 6; ++q) closed |= q < nb && (row[q] & present & all) == 0u;
+    return closed ? 0 : -1;
+}
+
 // ---------------------------------------------------------------------------
 // Multi-source face-connectivity test on label L around `cell` as seen by
 // transaction k: are all face branches of L (centre excluded when `excl`)
@@ -644,11 +764,10 @@ __device__ __forceinline__ uint32_t hslot(int cell) {
     return (uint32_t(cell) * 2654435761u) >> (32 - 12);
 }
 
-// returns 1 connected, 0 disconnected, -1 overflow; MODE picks the view of
+// returns 1 connected, 0 disconnected, -1 overflow; mode picks the view of
 // the undecided earlier transactions (view_label_b)
-template <int MODE>
-__device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32_t L, bool excl,
-                                 int k) {
+__device__ __forceinline__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32_t L,
+                                                 bool excl, int k, int mode) {
     const Dims& d = c.d;
     const int nface = c.dim3 ? 6 : 4;
     for (int h = threadIdx.x; h < BFS_HCAP; h += blockDim.x) S.hash[h] = ~0ull;
@@ -662,7 +781,7 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
         faces(d, cell, f);
         int nb = 0;
         for (int q = 0; q < nface; ++q) {
-            if (view_label_b<MODE>(f[q], k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
+            if (view_label_b(mode, f[q], k, L, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
             const int col = nb++;
             const unsigned long long val = ((unsigned long long)(uint32_t)f[q] << 8) | col;
             uint32_t h = hslot(f[q]);
@@ -690,35 +809,8 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
     int cur = 0;
     while (true) {
         if (threadIdx.x == 0 && S.result < 0) {
-            // union colours through recorded contacts
-            const int nb = S.nbranch;
-            int comp[6];
-            for (int q = 0; q < nb; ++q) comp[q] = q;
-            bool changed = true;
-            while (changed) {
-                changed = false;
-                for (int a = 0; a < nb; ++a)
-                    for (int b = 0; b < nb; ++b)
-                        if (((S.adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
-                            const int m = comp[a] < comp[b] ? comp[a] : comp[b];
-                            if (comp[a] != m || comp[b] != m) { comp[a] = m; comp[b] = m; changed = true; }
-                        }
-            }
-            bool all = true;
-            for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
-            if (S.overflow || S.fcount[cur] > BFS_FCAP) {
-                S.result = -2;
-            } else if (all) {
-                S.result = 1;
-            } else {
-                // a merged group with no frontier is closed: disconnected
-                const uint32_t present = S.pres[cur];
-                for (int q = 0; q < nb && S.result < 0; ++q) {
-                    bool has = false;
-                    for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1)) has = true;
-                    if (!has) S.result = 0;
-                }
-            }
+            if (S.overflow || S.fcount[cur] > BFS_FCAP) S.result = -2;
+            else S.result = colour_verdict(S.nbranch, S.adj, S.pres[cur]);
         }
         __syncthreads();
         if (S.result != -1) break;
@@ -731,14 +823,27 @@ __device__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c, int cell, int32
             const int col = S.frc[cur][e];
             int f[6];
             faces(d, cc, f);
-            for (int q = 0; q < nface; ++q) {
+            int32_t lab[6];
+            face_view_labels(c, mode, k, L, true, nface, f, lab);
+            // first hash probes of all faces issued together
+            bool want[6];
+            unsigned long long prv[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                want[q] = q < nface && !(excl && f[q] == cell) && lab[q] == L;
+                prv[q] = 0ull;
+                if (want[q])
+                    prv[q] = atomicCAS(&S.hash[hslot(f[q])], ~0ull, ((unsigned long long)(uint32_t)f[q] << 8) | col);
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                if (!want[q]) continue;
                 const int nbc = f[q];
-                if (excl && nbc == cell) continue;
-                if (view_label_b<MODE>(nbc, k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
                 const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | col;
                 uint32_t h = hslot(nbc);
+                unsigned long long prev = prv[q];
                 for (int probe = 0; probe < BFS_HCAP; ++probe) {
-                    const unsigned long long prev = atomicCAS(&S.hash[h], ~0ull, val);
+                    if (probe > 0) prev = atomicCAS(&S.hash[h], ~0ull, val);
                     if (prev == ~0ull) {
                         const int ins = atomicAdd(&S.inserted, 1);
                         const int pos = atomicAdd(&S.fcount[nxt], 1);
@@ -788,43 +893,31 @@ struct WaveBfs {
     uint8_t frc[2][WB_FCAP];
 };
 
-__device__ __forceinline__ unsigned long long wave_or64(unsigned long long v) {
+// OR over the wavefront of a value that few lanes hold (colour contacts):
+// the lanes holding one are visited through the ballot, each read with one
+// lane read per word
+__device__ __forceinline__ unsigned long long wave_or_sparse(unsigned long long v) {
+    unsigned long long any = __ballot(v != 0ull), acc = 0ull;
+    while (any) {
+        const int l = __builtin_ffsll((long long)any) - 1;
+        const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v)), l));
+        const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(v >> 32)), l));
+        acc |= ((unsigned long long)hi << 32) | lo;
+        any &= any - 1ull;
+    }
+    return acc;
+}
+
+// OR over the wavefront of a 6-bit colour set, one ballot per colour
+__device__ __forceinline__ uint32_t wave_or6(uint32_t v) {
+    uint32_t acc = 0u;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = (unsigned long long)__shfl_xor((long long)v, off);
-        v |= o;
-    }
-    return v;
+    for (int q = 0; q < 6; ++q) acc |= __ballot((v >> q) & 1u) != 0ull ? (1u << q) : 0u;
+    return acc;
 }
 
-// colours merged through recorded contacts; 1 all one group, 0 a group with
-// no frontier left, -1 undecided (uniform over the wavefront)
-__device__ __forceinline__ int colour_verdict(int nb, unsigned long long adj, uint32_t present) {
-    int comp[6];
-    for (int q = 0; q < nb; ++q) comp[q] = q;
-    bool changed = true;
-    while (changed) {
-        changed = false;
-        for (int a = 0; a < nb; ++a)
-            for (int b = 0; b < nb; ++b)
-                if (((adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
-                    const int m = comp[a] < comp[b] ? comp[a] : comp[b];
-                    comp[a] = m; comp[b] = m; changed = true;
-                }
-    }
-    bool all = true;
-    for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
-    if (all) return 1;
-    for (int q = 0; q < nb; ++q) {
-        bool has = false;
-        for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1u)) has = true;
-        if (!has) return 0;
-    }
-    return -1;
-}
-
-template <int MODE>
-__device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_t L, bool excl, int k) {
+__device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_t L, bool excl,
+                                                  int k, int mode) {
     const Dims& d = c.d;
     const int nface = c.dim3 ? 6 : 4;
     const int lane = threadIdx.x & 63;
@@ -837,7 +930,7 @@ __device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_
     int seed[6], nb = 0;
     unsigned long long adj = 0ull;
     for (int q = 0; q < nface; ++q) {
-        if (view_label_b<MODE>(f[q], k, L, c.phi, c.cand_id, c.state, c.Cold, c.Cnew) != L) continue;
+        if (view_label_b(mode, f[q], k, L, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
         seed[nb++] = f[q];
     }
     if (nb <= 1) return 1;
@@ -878,54 +971,49 @@ __device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_
             const int col = act ? int(S.frc[cur][e]) : 0;
             int g6[6];
             faces(d, cc, g6);
-            // the neighbours' labels in the view: the candidate ids and labels
-            // of all faces are loaded before any is used, then the (rare)
-            // earlier candidates among them are resolved
-            int cid[6];
             int32_t lab[6];
+            face_view_labels(c, mode, k, L, act, nface, g6, lab);
+            // first hash probes of all faces issued together; collisions then
+            // probe on, face by face
+            bool want[6], ins[6];
+            uint32_t hq[6];
+            unsigned long long prv[6];
+            const unsigned long long ctag = uint32_t(col);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                cid[q] = -1;
-                lab[q] = -1;
-                if (act && q < nface) {
-                    cid[q] = c.cand_id[g6[q]];
-                    lab[q] = c.phi[g6[q]];
-                }
+                want[q] = act && q < nface && !(excl && g6[q] == cell) && lab[q] == L;
+                ins[q] = false;
+                hq[q] = (uint32_t(g6[q]) * 2654435761u) >> (32 - 10);
+                prv[q] = 0ull;
+                if (want[q]) prv[q] = atomicCAS(&S.hash[hq[q]], ~0ull, ((unsigned long long)(uint32_t)g6[q] << 8) | ctag);
             }
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
-                const int j = cid[q];
-                if (j >= 0 && j < k) {
-                    const int sj = ld_acq(c.state + j);
-                    if (sj == ACCEPT) lab[q] = c.Cnew[j];
-                    else if (MODE == VIEW_UPPER && sj != REJECT && c.Cnew[j] == L) lab[q] = L;
-                    else if (MODE == VIEW_LOWER && sj != REJECT && c.Cold[j] == L) lab[q] = -1;
-                }
-            }
-            for (int q = 0; q < nface; ++q) {
+                if (!want[q]) continue;
                 const int nbc = g6[q];
-                bool ins = false;
-                if (act && !(excl && nbc == cell) && lab[q] == L) {
-                    const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | uint32_t(col);
-                    uint32_t h = (uint32_t(nbc) * 2654435761u) >> (32 - 10);
-                    int probe = 0;
-                    for (; probe < WB_HCAP; ++probe) {
-                        const unsigned long long prev = atomicCAS(&S.hash[h], ~0ull, val);
-                        if (prev == ~0ull) { ins = true; break; }
-                        if (int(prev >> 8) == nbc) {
-                            const int oc = int(prev & 0xffull);
-                            if (oc != col) myadj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
-                            break;
-                        }
-                        h = (h + 1) & (WB_HCAP - 1);
+                const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | ctag;
+                unsigned long long prev = prv[q];
+                uint32_t h = hq[q];
+                int probe = 1;
+                while (true) {
+                    if (prev == ~0ull) { ins[q] = true; break; }
+                    if (int(prev >> 8) == nbc) {
+                        const int oc = int(prev & 0xffull);
+                        if (oc != col) myadj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
+                        break;
                     }
-                    if (probe == WB_HCAP) ovf = true;
+                    if (probe++ == WB_HCAP) { ovf = true; break; }
+                    h = (h + 1) & (WB_HCAP - 1);
+                    prev = atomicCAS(&S.hash[h], ~0ull, val);
                 }
-                const unsigned long long bal = __ballot(ins);
+            }
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                const unsigned long long bal = __ballot(ins[q]);
                 const int pos = ncount + __popcll(bal & lt);
-                if (ins) {
+                if (ins[q]) {
                     if (pos < WB_FCAP) {
-                        S.fr[nxt][pos] = nbc;
+                        S.fr[nxt][pos] = g6[q];
                         S.frc[nxt][pos] = uint8_t(col);
                     }
                     mypres |= 1u << col;
@@ -934,8 +1022,8 @@ __device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_
             }
         }
         inserted += ncount;
-        adj |= wave_or64(myadj);
-        present = uint32_t(wave_or64(mypres));
+        adj |= wave_or_sparse(myadj);
+        present = wave_or6(mypres);
         if (__ballot(ovf) != 0ull || ncount > WB_FCAP || inserted > WB_HMAX) return -1;
         nfront = ncount;
         cur = nxt;
@@ -946,7 +1034,10 @@ __device__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, int cell, int32_
 // One wavefront per queued candidate (WB_WAVES per workgroup).  Exact turns
 // and bounds searches as k_flip_bfs; a search that outgrows the wavefront's
 // hash queues the candidate (still QUEUED) for k_flip_bfs's workgroup
-// search.  Block 0 clears the counts that the next round fills.
+// search.  Block 0 clears the counts that the next round fills.  (Entries are
+// assigned statically: a wavefront taking the next entry through a global
+// counter measured slower, the counter's atomics costing more than the
+// balance gained.)
 __global__ void __launch_bounds__(256) k_flip_bfs_wave(FlipCtx c, int round, int* __restrict__ ovq,
                                                         int* __restrict__ ovcount) {
     __shared__ WaveBfs S[WB_WAVES];
@@ -960,27 +1051,10 @@ __global__ void __launch_bounds__(256) k_flip_bfs_wave(FlipCtx c, int round, int
         const int k = c.queue[round & 1][qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
-        int verdict;    // ACCEPT, REJECT, RETRY, or -1: to the workgroup search
-        if (!(flags & 4)) {
-            int ok = 1;
-            if (flags & 1) ok = bfs_connected_wave<VIEW_EXACT>(S[w], c, cell, c.Cold[k], true, k);
-            if (ok == 1 && (flags & 2)) ok = bfs_connected_wave<VIEW_EXACT>(S[w], c, cell, c.Cnew[k], false, k);
-            verdict = ok < 0 ? -1 : (ok ? ACCEPT : REJECT);
-        } else {
-            bool fail = false, open = false, big = false;
-            for (int t = 0; t < 2 && !fail; ++t) {
-                if (!(flags & (1 << t))) continue;
-                const int32_t L = t == 0 ? c.Cold[k] : c.Cnew[k];
-                const bool excl = t == 0;
-                const int up = bfs_connected_wave<VIEW_UPPER>(S[w], c, cell, L, excl, k);
-                if (up == 0) { fail = true; break; }
-                if (up < 0) { big = true; continue; }
-                const int lo = bfs_connected_wave<VIEW_LOWER>(S[w], c, cell, L, excl, k);
-                if (lo < 0) big = true;
-                else if (lo != 1) open = true;
-            }
-            verdict = fail ? REJECT : (big ? -1 : (open ? RETRY : ACCEPT));
-        }
+        // ACCEPT, REJECT, RETRY, or -1: to the workgroup search
+        const int verdict = flip_verdict(flags, false, [&](int mode, int t) {
+            return bfs_connected_wave(S[w], c, cell, t == 0 ? c.Cold[k] : c.Cnew[k], t == 0, k, mode);
+        });
         if (lane == 0) {
             if (verdict == ACCEPT || verdict == REJECT) decide(c, k, verdict);
             else if (verdict == RETRY) st_rel(c.state + k, RETRY);
@@ -1030,7 +1104,7 @@ __device__ void flip_bfs_fallback(FlipCtx& c, const int* __restrict__ fbq, int* 
                 faces(d, cell, f);
                 int nb = 0;
                 for (int q = 0; q < nface; ++q) {
-                    if (view_label(f[q], k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                    if (view_label(f[q], k, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
                     const int col = nb++;
                     int oc = 0;
                     if (claim_mark(mark, f[q], tag, col, &oc)) {
@@ -1045,30 +1119,7 @@ __device__ void flip_bfs_fallback(FlipCtx& c, const int* __restrict__ fbq, int* 
             int cur = 0;
             while (true) {
                 if (threadIdx.x == 0 && result < 0) {
-                    const int nb = nbranch;
-                    int comp[6];
-                    for (int q = 0; q < nb; ++q) comp[q] = q;
-                    bool changed = true;
-                    while (changed) {
-                        changed = false;
-                        for (int a = 0; a < nb; ++a)
-                            for (int b = 0; b < nb; ++b)
-                                if (((adj >> (a * 8 + b)) & 1ull) && comp[b] != comp[a]) {
-                                    const int m = comp[a] < comp[b] ? comp[a] : comp[b];
-                                    comp[a] = m; comp[b] = m; changed = true;
-                                }
-                    }
-                    bool all = true;
-                    for (int q = 0; q < nb; ++q) all &= comp[q] == comp[0];
-                    if (all) result = 1;
-                    else {
-                        const uint32_t present = pres[cur];
-                        for (int q = 0; q < nb && result < 0; ++q) {
-                            bool has = false;
-                            for (int p = 0; p < nb; ++p) if (comp[p] == comp[q] && ((present >> p) & 1u)) has = true;
-                            if (!has) result = 0;
-                        }
-                    }
+                    result = colour_verdict(nbranch, adj, pres[cur]);
                     fcount[cur ^ 1] = 0;
                     pres[cur ^ 1] = 0u;
                 }
@@ -1088,7 +1139,7 @@ __device__ void flip_bfs_fallback(FlipCtx& c, const int* __restrict__ fbq, int* 
                     for (int q = 0; q < nface; ++q) {
                         const int nbc = f[q];
                         if (excl && nbc == cell) continue;
-                        if (view_label(nbc, k, c.phi, c.cand_id, c.state, c.Cnew) != L) continue;
+                        if (view_label(nbc, k, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
                         int oc = 0;
                         if (claim_mark(mark, nbc, tg, col, &oc)) {
                             frn[atomicAdd(&fcount[nxt], 1)] = nbc;
@@ -1126,37 +1177,19 @@ __global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, const int* __restri
         const int k = queue[qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
-        if (!(flags & 4)) {
-            // exact turn: every earlier transaction on the tested labels is decided
-            int ok = 1;
-            if (flags & 1) ok = bfs_connected_lds<VIEW_EXACT>(S, c, cell, c.Cold[k], true, k);
-            if (ok == 1 && (flags & 2)) ok = bfs_connected_lds<VIEW_EXACT>(S, c, cell, c.Cnew[k], false, k);
-            if (threadIdx.x == 0) {
-                if (ok < 0) {
-                    const int p = atomicAdd(&c.s->fbcount, 1);
-                    st_rel(fbq + p, k);
-                } else {
-                    decide(c, k, ok ? ACCEPT : REJECT);
-                }
-            }
-        } else {
-            // bounds: a test fails if its branches are disconnected in the
-            // upper view, passes if connected in the lower view; one failed
-            // test refuses, all passed accept, anything else waits (RETRY)
-            bool fail = false, open = false;
-            for (int t = 0; t < 2 && !fail; ++t) {
-                if (!(flags & (1 << t))) continue;
-                const int32_t L = t == 0 ? c.Cold[k] : c.Cnew[k];
-                const bool excl = t == 0;
-                const int up = bfs_connected_lds<VIEW_UPPER>(S, c, cell, L, excl, k);
-                if (up == 0) { fail = true; break; }
-                if (up < 0) { open = true; continue; }
-                if (bfs_connected_lds<VIEW_LOWER>(S, c, cell, L, excl, k) != 1) open = true;
-            }
-            if (threadIdx.x == 0) {
-                if (fail) decide(c, k, REJECT);
-                else if (!open) decide(c, k, ACCEPT);
-                else st_rel(c.state + k, RETRY);
+        // an exact turn whose search outgrows this visited set goes to the
+        // fallback queue; a bounds turn whose search does waits (RETRY)
+        const int verdict = flip_verdict(flags, true, [&](int mode, int t) {
+            return bfs_connected_lds(S, c, cell, t == 0 ? c.Cold[k] : c.Cnew[k], t == 0, k, mode);
+        });
+        if (threadIdx.x == 0) {
+            if (verdict < 0) {
+                const int p = atomicAdd(&c.s->fbcount, 1);
+                st_rel(fbq + p, k);
+            } else if (verdict == RETRY) {
+                st_rel(c.state + k, RETRY);
+            } else {
+                decide(c, k, verdict);
             }
         }
         __syncthreads();
@@ -1173,19 +1206,25 @@ __global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, const int* __restri
     }
 }
 
+// every candidate's old label back in phi (a transaction abandoned on error)
+__global__ void k_flip_restore(int m, const int* __restrict__ Ccell, const int* __restrict__ Cold,
+                               int32_t* __restrict__ phi) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) phi[Ccell[k]] = Cold[k];
+}
+
 // apply the decided transactions (src/gpk_transitions.cpp:300-312)
 template <class R>
 __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __restrict__ Cold,
                              const int* __restrict__ Cnew, const R* __restrict__ Cg,
                              const R* __restrict__ Canew, const int* __restrict__ state,
                              int32_t* __restrict__ phi, R* __restrict__ g, R* __restrict__ alpha,
-                             int* __restrict__ count, int* __restrict__ cand_id, Scalars* s,
+                             int* __restrict__ count, Scalars* s,
                              int all_accept, R* __restrict__ rho, R rho_rex, int sweep) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     bool acc = false, def = false;
     if (k < m) {
         const int cell = Ccell[k];
-        cand_id[cell] = -1;
         if (all_accept || state[k] == ACCEPT) {
             phi[cell] = Cnew[k];
             g[cell] = Cg[k];
@@ -1195,6 +1234,7 @@ __global__ void k_flip_apply(int m, const int* __restrict__ Ccell, const int* __
             atomicAdd(count + Cnew[k], 1);
             acc = true;
         } else {
+            phi[cell] = Cold[k];   // the cell's label word back from ~k
             const R an = Canew[k];
             alpha[cell] = an < Eps<R>::flip ? Eps<R>::flip : an;
             def = true;
