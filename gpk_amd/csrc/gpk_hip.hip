@@ -1186,10 +1186,10 @@ struct Engine final : EngineBase {
             reserve_tags(2LL * m);
             int round = 0;
             // one wavefront per queued search (four per workgroup, three
-            // 43 KB workgroups per CU), then the workgroup search for those
+            // 52 KB workgroups per CU), then the workgroup search for the few
             // that outgrew a wavefront's hash
             const int wave_grid = std::max(1, std::min((m + WB_WAVES - 1) / WB_WAVES, 768));
-            const int bfs_grid = std::max(1, std::min(m, 384));
+            const int bfs_grid = std::max(1, std::min(m, 128));
             // the first batch is as long as the last step's transaction: rounds
             // past the end are cheap no-ops, host checks are not
             int batch = flip_batch;

@@ -877,18 +877,23 @@ __device__ __forceinline__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c,
 
 
 // The same test run by one wavefront, for the small components most
-// searches meet: a 1024-slot LDS hash of visited cells, frontiers of up to
-// 256 cells, the frontier appends ordered by ballot prefix within the
-// wavefront, colour contacts and presence held in registers (uniform over
-// the wavefront), so no workgroup barrier is needed.  Returns 1 connected,
-// 0 disconnected, -1 when the search outgrows the hash or a frontier.
-constexpr int WB_HCAP = 1024;       // hash slots (power of two)
-constexpr int WB_HMAX = 768;        // insertions before overflow
-constexpr int WB_FCAP = 256;        // frontier capacity
+// searches meet: a 2048-slot LDS hash of visited cells (32-bit entries: cell
+// << 3 | colour, so grids below 2^29 cells), frontiers of up to 512 cells,
+// the frontier appends ordered by ballot prefix within the wavefront, colour
+// contacts and presence held in registers (uniform over the wavefront), so
+// no workgroup barrier is needed.  Returns 1 connected, 0 disconnected, -1
+// when the search outgrows the hash or a frontier.  13 KB per wavefront:
+// three 52 KB workgroups per CU.
+constexpr int WB_HCAP = 2048;       // hash slots (power of two)
+constexpr int WB_HBITS = 11;        // log2(WB_HCAP)
+constexpr int WB_HMAX = 1536;       // insertions before overflow
+constexpr int WB_FCAP = 512;        // frontier capacity
 constexpr int WB_WAVES = 4;         // wavefronts (searches) per 256-thread workgroup
+constexpr int WB_MAXN = 1 << 29;    // grids this hash can key
+constexpr uint32_t WB_EMPTY = 0xffffffffu;
 
 struct WaveBfs {
-    unsigned long long hash[WB_HCAP];   // (cell << 8) | colour, ~0 empty
+    alignas(16) uint32_t hash[WB_HCAP];   // (cell << 3) | colour, WB_EMPTY empty
     int fr[2][WB_FCAP];
     uint8_t frc[2][WB_FCAP];
 };
@@ -922,7 +927,8 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
     const int nface = c.dim3 ? 6 : 4;
     const int lane = threadIdx.x & 63;
     const unsigned long long lt = (1ull << lane) - 1ull;
-    for (int h = lane; h < WB_HCAP; h += 64) S.hash[h] = ~0ull;
+    for (int h = lane; h < WB_HCAP / 4; h += 64)
+        reinterpret_cast<uint4*>(S.hash)[h] = make_uint4(WB_EMPTY, WB_EMPTY, WB_EMPTY, WB_EMPTY);
     // seeds: the face neighbours of label L, one colour each in face order (a
     // neighbour met twice, on grids of side 2, is one cell of two colours)
     int f[6];
@@ -945,9 +951,9 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
             continue;
         }
         if (lane == 0) {
-            uint32_t h = (uint32_t(seed[q]) * 2654435761u) >> (32 - 10);
-            while (S.hash[h] != ~0ull) h = (h + 1) & (WB_HCAP - 1);
-            S.hash[h] = ((unsigned long long)(uint32_t)seed[q] << 8) | uint32_t(q);
+            uint32_t h = (uint32_t(seed[q]) * 2654435761u) >> (32 - WB_HBITS);
+            while (S.hash[h] != WB_EMPTY) h = (h + 1) & (WB_HCAP - 1);
+            S.hash[h] = (uint32_t(seed[q]) << 3) | uint32_t(q);
             S.fr[0][nfront] = seed[q];
             S.frc[0][nfront] = uint8_t(q);
         }
@@ -976,35 +982,34 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
             // first hash probes of all faces issued together; collisions then
             // probe on, face by face
             bool want[6], ins[6];
-            uint32_t hq[6];
-            unsigned long long prv[6];
-            const unsigned long long ctag = uint32_t(col);
+            uint32_t hq[6], prv[6];
+            const uint32_t ctag = uint32_t(col);
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 want[q] = act && q < nface && !(excl && g6[q] == cell) && lab[q] == L;
                 ins[q] = false;
-                hq[q] = (uint32_t(g6[q]) * 2654435761u) >> (32 - 10);
-                prv[q] = 0ull;
-                if (want[q]) prv[q] = atomicCAS(&S.hash[hq[q]], ~0ull, ((unsigned long long)(uint32_t)g6[q] << 8) | ctag);
+                hq[q] = (uint32_t(g6[q]) * 2654435761u) >> (32 - WB_HBITS);
+                prv[q] = 0u;
+                if (want[q]) prv[q] = atomicCAS(&S.hash[hq[q]], WB_EMPTY, (uint32_t(g6[q]) << 3) | ctag);
             }
 #pragma unroll
             for (int q = 0; q < 6; ++q) {
                 if (!want[q]) continue;
                 const int nbc = g6[q];
-                const unsigned long long val = ((unsigned long long)(uint32_t)nbc << 8) | ctag;
-                unsigned long long prev = prv[q];
+                const uint32_t val = (uint32_t(nbc) << 3) | ctag;
+                uint32_t prev = prv[q];
                 uint32_t h = hq[q];
                 int probe = 1;
                 while (true) {
-                    if (prev == ~0ull) { ins[q] = true; break; }
-                    if (int(prev >> 8) == nbc) {
-                        const int oc = int(prev & 0xffull);
+                    if (prev == WB_EMPTY) { ins[q] = true; break; }
+                    if (int(prev >> 3) == nbc) {
+                        const int oc = int(prev & 7u);
                         if (oc != col) myadj |= (1ull << (oc * 8 + col)) | (1ull << (col * 8 + oc));
                         break;
                     }
                     if (probe++ == WB_HCAP) { ovf = true; break; }
                     h = (h + 1) & (WB_HCAP - 1);
-                    prev = atomicCAS(&S.hash[h], ~0ull, val);
+                    prev = atomicCAS(&S.hash[h], WB_EMPTY, val);
                 }
             }
 #pragma unroll
@@ -1051,8 +1056,10 @@ __global__ void __launch_bounds__(256) k_flip_bfs_wave(FlipCtx c, int round, int
         const int k = c.queue[round & 1][qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
-        // ACCEPT, REJECT, RETRY, or -1: to the workgroup search
+        // ACCEPT, REJECT, RETRY, or -1: to the workgroup search (every search
+        // of a grid too large for the wavefront hash's keys goes there)
         const int verdict = flip_verdict(flags, false, [&](int mode, int t) {
+            if (c.d.N >= WB_MAXN) return -1;
             return bfs_connected_wave(S[w], c, cell, t == 0 ? c.Cold[k] : c.Cnew[k], t == 0, k, mode);
         });
         if (lane == 0) {
