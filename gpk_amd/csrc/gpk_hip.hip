@@ -1173,7 +1173,8 @@ struct Engine final : EngineBase {
             k_cand_pos<<<nblk(2 * m, 256), 256, 0, stream>>>(2 * m, Ps.as<unsigned long long>(),
                                                               Cold.as<int>(), pos_old.as<int>(),
                                                               pos_new.as<int>());
-            set_ints({{&ds()->undecided, m}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0},
+            k_fill_i32<<<1, 64, 0, stream>>>(16, &ds()->qcount[0][0], 0);
+            set_ints({{&ds()->undecided, m}, {&ds()->fbcount, 0},
                       {&ds()->ovcount[0], 0}, {&ds()->ovcount[1], 0}});
             FlipCtx fc;
             fc.d = dims; fc.dim3 = dim3; fc.m = m;
@@ -1182,13 +1183,14 @@ struct Engine final : EngineBase {
             fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>();
             fc.pos_new = pos_new.as<int>(); fc.count = count.as<int>();
             fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
-            fc.qflags = qflags.as<int>(); fc.s = ds();
+            fc.qflags = qflags.as<int>(); fc.s = ds(); fc.slab = (m + 7) / 8;
             reserve_tags(2LL * m);
             int round = 0;
             // one wavefront per queued search (four per workgroup, three
             // 52 KB workgroups per CU), then the workgroup search for the few
             // that outgrew a wavefront's hash
-            const int wave_grid = std::max(1, std::min((m + WB_WAVES - 1) / WB_WAVES, 768));
+            // a multiple of 8 workgroups: every band slab has its own
+            const int wave_grid = 8 * std::max(1, std::min((m + 8 * WB_WAVES - 1) / (8 * WB_WAVES), 96));
             const int bfs_grid = std::max(1, std::min(m, 128));
             // the first batch is as long as the last step's transaction: rounds
             // past the end are cheap no-ops, host checks are not
@@ -1472,17 +1474,17 @@ struct Engine final : EngineBase {
         int po = ol < nl ? 0 : 1, pn = 1 - po;
         HIPCHK(hipMemcpyAsync(pos_old.p, &po, 4, hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(pos_new.p, &pn, 4, hipMemcpyHostToDevice, stream));
-        set_ints({{&ds()->undecided, 1}, {&ds()->qcount[0], 0}, {&ds()->qcount[1], 0}, {&ds()->fbcount, 0}});
+        set_ints({{&ds()->undecided, 1}, {&ds()->qcount[0][0], 0}, {&ds()->qcount[1][0], 0}, {&ds()->fbcount, 0}});
         FlipCtx fc;
         fc.d = dims; fc.dim3 = dim3; fc.m = 1;
         fc.Ccell = Ccell.as<int>(); fc.Cold = Cold.as<int>(); fc.Cnew = Cnew.as<int>();
         fc.state = state.as<int>(); fc.phi = phi.as<int32_t>();
         fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>(); fc.pos_new = pos_new.as<int>();
         fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
-        fc.qflags = qflags.as<int>(); fc.s = ds();
+        fc.qflags = qflags.as<int>(); fc.s = ds(); fc.slab = 1;
         reserve_tags(2);
         k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
-        k_flip_bfs<<<1, 256, 0, stream>>>(fc, queue0.as<int>(), &ds()->qcount[0], fbq.as<int>(), mark.as<int>(),
+        k_flip_bfs<<<1, 256, 0, stream>>>(fc, queue0.as<int>(), &ds()->qcount[0][0], fbq.as<int>(), mark.as<int>(),
                                           reinterpret_cast<int*>(work3.p),
                                           reinterpret_cast<int*>(work4.p), tagctr.as<int>());
         int st = 0;

@@ -296,7 +296,7 @@ struct Scalars {
     int band2;                      // size of the latest recomputed band
     int ncand;                      // flip candidates
     int undecided;                  // undecided transactions
-    int qcount[2];                  // BFS queues (double-buffered per round)
+    int qcount[2][8];               // search queues per band slab (double-buffered per round)
     int ovcount[2];                 // wavefront searches handed to the workgroup search, per round
     int fbcount;                    // BFS fallback queue
     int accepted, deferred;
@@ -511,6 +511,7 @@ struct FlipCtx {
     const int* pos_old; const int* pos_new;
     const int* count;          // label counts at the start of the flip pass
     int* queue[2]; int* qflags;
+    int slab;                  // candidates per band slab: queue segment x holds slab x's entries
     Scalars* s;
 };
 
@@ -661,9 +662,21 @@ __global__ void __launch_bounds__(256) k_flip_round(FlipCtx c, int round) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= c.m) return;
     int prev;
-    if (flip_try(c, k, round, &prev) == 2) {
-        const int qi = atomicAdd(&c.s->qcount[round & 1], 1);
-        c.queue[round & 1][qi] = k;
+    const bool queued = flip_try(c, k, round, &prev) == 2;
+    // queue entries reserved per slab with one atomic per slab met by the
+    // wavefront (its candidates are consecutive: one slab, or two)
+    const int x = k / c.slab;
+    const unsigned long long lt = (1ull << (threadIdx.x & 63)) - 1ull;
+    unsigned long long todo = __ballot(queued);
+    while (todo) {
+        const int leader = __builtin_ffsll((long long)todo) - 1;
+        const int xl = __shfl(x, leader);
+        const unsigned long long grp = __ballot(queued && x == xl);
+        int base = 0;
+        if ((threadIdx.x & 63) == leader) base = atomicAdd(&c.s->qcount[round & 1][xl], __popcll(grp));
+        base = __shfl(base, leader);
+        if (queued && x == xl) c.queue[round & 1][xl * c.slab + base + __popcll(grp & lt)] = k;
+        todo &= ~grp;
     }
 }
 
@@ -1036,7 +1049,11 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
     }
 }
 
-// One wavefront per queued candidate (WB_WAVES per workgroup).  Exact turns
+// One wavefront per queued candidate (WB_WAVES per workgroup).  Candidates
+// are numbered in band (grid) order and queued per slab of the band, one
+// slab for each of the eight XCDs: workgroups are dispatched to the XCDs in
+// turn, so workgroup b searches slab b % 8 and each XCD's L2 holds the labels
+// of one region of the grid (the grid size is a multiple of 8).  Exact turns
 // and bounds searches as k_flip_bfs; a search that outgrows the wavefront's
 // hash queues the candidate (still QUEUED) for k_flip_bfs's workgroup
 // search.  Block 0 clears the counts that the next round fills.  (Entries are
@@ -1047,13 +1064,13 @@ __global__ void __launch_bounds__(256) k_flip_bfs_wave(FlipCtx c, int round, int
                                                         int* __restrict__ ovcount) {
     __shared__ WaveBfs S[WB_WAVES];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        c.s->qcount[(round + 1) & 1] = 0;
-        ovcount[(round + 1) & 1] = 0;
-    }
-    const int n = c.s->qcount[round & 1];
-    for (int qi = blockIdx.x * WB_WAVES + w; qi < n; qi += gridDim.x * WB_WAVES) {
-        const int k = c.queue[round & 1][qi];
+    if (blockIdx.x == 0 && threadIdx.x < 8) c.s->qcount[(round + 1) & 1][threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ovcount[(round + 1) & 1] = 0;
+    const int x = blockIdx.x & 7, nbx = gridDim.x >> 3;
+    const int n = c.s->qcount[round & 1][x];
+    const int* q = c.queue[round & 1] + x * c.slab;
+    for (int qi = (blockIdx.x >> 3) * WB_WAVES + w; qi < n; qi += nbx * WB_WAVES) {
+        const int k = q[qi];
         const int flags = c.qflags[k];
         const int cell = c.Ccell[k];
         // ACCEPT, REJECT, RETRY, or -1: to the workgroup search (every search
