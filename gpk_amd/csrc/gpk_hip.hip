@@ -1186,11 +1186,11 @@ struct Engine final : EngineBase {
             fc.qflags = qflags.as<int>(); fc.s = ds(); fc.slab = (m + 7) / 8;
             reserve_tags(2LL * m);
             int round = 0;
-            // one wavefront per queued search (four per workgroup, three
-            // 52 KB workgroups per CU), then the workgroup search for the few
-            // that outgrew a wavefront's hash
-            // a multiple of 8 workgroups: every band slab has its own
-            const int wave_grid = 8 * std::max(1, std::min((m + 8 * WB_WAVES - 1) / (8 * WB_WAVES), 96));
+            // one wavefront per queued search (four per workgroup, four 40 KB
+            // workgroups per CU), then the workgroup search for the few that
+            // outgrew a wavefront's hash; a multiple of 8 workgroups, so that
+            // every band slab has its own
+            const int wave_grid = 8 * std::max(1, std::min((m + 8 * WB_WAVES - 1) / (8 * WB_WAVES), 128));
             const int bfs_grid = std::max(1, std::min(m, 128));
             // the first batch is as long as the last step's transaction: rounds
             // past the end are cheap no-ops, host checks are not

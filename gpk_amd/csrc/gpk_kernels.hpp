@@ -891,24 +891,23 @@ __device__ __forceinline__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c,
 
 // The same test run by one wavefront, for the small components most
 // searches meet: a 2048-slot LDS hash of visited cells (32-bit entries: cell
-// << 3 | colour, so grids below 2^29 cells), frontiers of up to 512 cells,
+// << 3 | colour, so grids below 2^29 cells), frontiers of up to 256 cells,
 // the frontier appends ordered by ballot prefix within the wavefront, colour
 // contacts and presence held in registers (uniform over the wavefront), so
 // no workgroup barrier is needed.  Returns 1 connected, 0 disconnected, -1
-// when the search outgrows the hash or a frontier.  13 KB per wavefront:
-// three 52 KB workgroups per CU.
+// when the search outgrows the hash or a frontier.  10 KB per wavefront:
+// four 40 KB workgroups per CU.
 constexpr int WB_HCAP = 2048;       // hash slots (power of two)
 constexpr int WB_HBITS = 11;        // log2(WB_HCAP)
 constexpr int WB_HMAX = 1536;       // insertions before overflow
-constexpr int WB_FCAP = 512;        // frontier capacity
+constexpr int WB_FCAP = 256;        // frontier capacity
 constexpr int WB_WAVES = 4;         // wavefronts (searches) per 256-thread workgroup
 constexpr int WB_MAXN = 1 << 29;    // grids this hash can key
 constexpr uint32_t WB_EMPTY = 0xffffffffu;
 
 struct WaveBfs {
     alignas(16) uint32_t hash[WB_HCAP];   // (cell << 3) | colour, WB_EMPTY empty
-    int fr[2][WB_FCAP];
-    uint8_t frc[2][WB_FCAP];
+    uint32_t fr[2][WB_FCAP];   // (cell << 3) | colour
 };
 
 // OR over the wavefront of a value that few lanes hold (colour contacts):
@@ -967,8 +966,7 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
             uint32_t h = (uint32_t(seed[q]) * 2654435761u) >> (32 - WB_HBITS);
             while (S.hash[h] != WB_EMPTY) h = (h + 1) & (WB_HCAP - 1);
             S.hash[h] = (uint32_t(seed[q]) << 3) | uint32_t(q);
-            S.fr[0][nfront] = seed[q];
-            S.frc[0][nfront] = uint8_t(q);
+            S.fr[0][nfront] = (uint32_t(seed[q]) << 3) | uint32_t(q);
         }
         ++nfront;
         present |= 1u << q;
@@ -986,8 +984,9 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
         for (int e0 = 0; e0 < nfront; e0 += 64) {
             const int e = e0 + lane;
             const bool act = e < nfront;
-            const int cc = act ? S.fr[cur][e] : 0;
-            const int col = act ? int(S.frc[cur][e]) : 0;
+            const uint32_t fe = act ? S.fr[cur][e] : 0u;
+            const int cc = int(fe >> 3);
+            const int col = int(fe & 7u);
             int g6[6];
             faces(d, cc, g6);
             int32_t lab[6];
@@ -1031,8 +1030,7 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
                 const int pos = ncount + __popcll(bal & lt);
                 if (ins[q]) {
                     if (pos < WB_FCAP) {
-                        S.fr[nxt][pos] = g6[q];
-                        S.frc[nxt][pos] = uint8_t(col);
+                        S.fr[nxt][pos] = (uint32_t(g6[q]) << 3) | ctag;
                     }
                     mypres |= 1u << col;
                 }
