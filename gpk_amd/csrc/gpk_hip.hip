@@ -848,7 +848,10 @@ struct Engine final : EngineBase {
     // ---- interface mask (src/gpk_geometry.cpp:42-82); mask_fresh records
     // that mask matches the current labels, so a compaction can skip it
     void update_mask() {
-        k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
+        if (dims.Nx % 4 == 0)
+            k_mask4<<<gsblk(N / 4, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
+        else
+            k_mask<<<gsblk(N, 256), 256, 0, stream>>>(dims, phi.as<int32_t>(), mask.as<uint8_t>());
         mask_fresh = true;
     }
 
@@ -872,7 +875,14 @@ struct Engine final : EngineBase {
     void mark_dirty(int n, int ncell, const int* c0, const int* c1, const int* st) {
         if (n <= 0) return;
         const long long W = 2 * remap_r + 1;
-        const long long total = (long long)n * ncell * (dim3 ? W * W * W : W * W);
+        const long long rows = (long long)n * ncell * (dim3 ? W * W : W);
+        if (rows < (1LL << 31)) {
+            const int grid = int(std::max(1LL, std::min<long long>((rows + 255) / 256, 65536)));
+            k_mark_dirty_rows<<<grid, 256, 0, stream>>>(dims, dim3, remap_r, n, ncell, c0, c1, st,
+                                                          rs_dirty.as<uint8_t>());
+            return;
+        }
+        const long long total = rows * W;
         const int grid = int(std::max(1LL, std::min<long long>((total + 255) / 256, 65536)));
         k_mark_dirty<<<grid, 256, 0, stream>>>(dims, dim3, remap_r, n, ncell, c0, c1, st,
                                                  rs_dirty.as<uint8_t>());

@@ -49,6 +49,35 @@ __global__ void k_mask(Dims d, const int32_t* __restrict__ phi, uint8_t* __restr
     }
 }
 
+// The same mask four cells at a time along x (Nx a multiple of 4): the four
+// labels of a group and of its four face-neighbour groups in y and z are
+// 16-byte loads, the two x neighbours outside the group two 4-byte loads, and
+// the four mask bytes one 4-byte store.
+__global__ void k_mask4(Dims d, const int32_t* __restrict__ phi, uint8_t* __restrict__ mask) {
+    const int qx = d.Nx >> 2;
+    const int nq = d.N >> 2;
+    for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < nq; g += gridDim.x * blockDim.x) {
+        const int row = g / qx;                   // z * Ny + y
+        const int x0 = (g - row * qx) << 2;
+        const int z = row / d.Ny, y = row - z * d.Ny;
+        const int ym = y == 0 ? d.Ny - 1 : y - 1, yp = y == d.Ny - 1 ? 0 : y + 1;
+        const int zm = z == 0 ? d.Nz - 1 : z - 1, zp = z == d.Nz - 1 ? 0 : z + 1;
+        const int base = row * d.Nx;
+        const int4 c = *reinterpret_cast<const int4*>(phi + base + x0);
+        const int4 a = *reinterpret_cast<const int4*>(phi + (z * d.Ny + ym) * d.Nx + x0);
+        const int4 b = *reinterpret_cast<const int4*>(phi + (z * d.Ny + yp) * d.Nx + x0);
+        const int4 e = *reinterpret_cast<const int4*>(phi + (zm * d.Ny + y) * d.Nx + x0);
+        const int4 f = *reinterpret_cast<const int4*>(phi + (zp * d.Ny + y) * d.Nx + x0);
+        const int32_t l = phi[base + (x0 == 0 ? d.Nx - 1 : x0 - 1)];
+        const int32_t r = phi[base + (x0 + 4 == d.Nx ? 0 : x0 + 4)];
+        const uint32_t m0 = (c.x != l) | (c.x != c.y) | (c.x != a.x) | (c.x != b.x) | (c.x != e.x) | (c.x != f.x);
+        const uint32_t m1 = (c.y != c.x) | (c.y != c.z) | (c.y != a.y) | (c.y != b.y) | (c.y != e.y) | (c.y != f.y);
+        const uint32_t m2 = (c.z != c.y) | (c.z != c.w) | (c.z != a.z) | (c.z != b.z) | (c.z != e.z) | (c.z != f.z);
+        const uint32_t m3 = (c.w != c.z) | (c.w != r) | (c.w != a.w) | (c.w != b.w) | (c.w != e.w) | (c.w != f.w);
+        *reinterpret_cast<uint32_t*>(mask + base + x0) = m0 | (m1 << 8) | (m2 << 16) | (m3 << 24);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Ordered stream compaction: out = indices i (ascending) with flags[i] != 0.
 // Block b owns items [4096 b, 4096 b + 4096), thread t the 16 consecutive
@@ -1632,6 +1661,28 @@ __global__ void k_mark_dirty(Dims d, int dim3, int r, int n, int ncell, const in
         decode(d, cell, z0, y0, x0);
         const int dz = dim3 ? o / (W * W) - r : 0, dy = (o / W) % W - r, dx = o % W - r;
         dirty[lin(d, wrapi(z0 + dz, d.Nz), wrapi(y0 + dy, d.Ny), wrapi(x0 + dx, d.Nx))] = 1;
+    }
+}
+
+// The same marks one box row (W cells along x) per thread, with 32-bit
+// index arithmetic (the host launches it when n * ncell * rows < 2^31).
+__global__ void k_mark_dirty_rows(Dims d, int dim3, int r, int n, int ncell, const int* __restrict__ c0,
+                                  const int* __restrict__ c1, const int* __restrict__ st,
+                                  uint8_t* __restrict__ dirty) {
+    const int W = 2 * r + 1;
+    const int rows = dim3 ? W * W : W;
+    const int total = n * ncell * rows;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < total; t += gridDim.x * blockDim.x) {
+        const int e = t / rows;
+        const int o = t - e * rows;
+        const int k = e / ncell;
+        if (st && st[k] != ACCEPT) continue;
+        const int cell = e - k * ncell == 0 ? c0[k] : c1[k];
+        int z0, y0, x0;
+        decode(d, cell, z0, y0, x0);
+        const int dz = dim3 ? o / W - r : 0, dy = (dim3 ? o % W : o) - r;
+        const int base = lin(d, wrapi(z0 + dz, d.Nz), wrapi(y0 + dy, d.Ny), 0);
+        for (int dx = -r; dx <= r; ++dx) dirty[base + wrapi(x0 + dx, d.Nx)] = 1;
     }
 }
 
