@@ -1080,7 +1080,7 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
 // are numbered in band (grid) order and queued per slab of the band, one
 // slab for each of the eight XCDs: workgroups are dispatched to the XCDs in
 // turn, so workgroup b searches slab b % 8 and each XCD's L2 holds the labels
-// of one region of the grid (the grid size is a multiple of 8).  Exact turns
+// of one region of the grid (the launch has a multiple of 8 workgroups).  Exact turns
 // and bounds searches as k_flip_bfs; a search that outgrows the wavefront's
 // hash queues the candidate (still QUEUED) for k_flip_bfs's workgroup
 // search.  Block 0 clears the counts that the next round fills.  (Entries are
