@@ -181,6 +181,15 @@ def test_anisotropic_growth_2d_labels_and_tolerance():
         assert np.all(np.abs(vo - vg) <= 1e-5 * np.maximum(np.abs(vo).max(), 1e-30)), s
 
 
+@pytest.mark.parametrize("dims,mode", [((18, 22, 30), "strict32"), ((18, 22, 30), "fp64"), ((1, 54, 70), "fp32")])
+def test_grain_growth_x_not_multiple_of_4(dims, mode):
+    # Nx = 30 / 70: the interface mask takes its one-cell-per-thread form (the
+    # four-cells-per-thread form needs Nx a multiple of 4)
+    phi, g = O.voronoi(*dims, 24, 9)
+    o, G = pair(dims, _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 15, check_every=3)
+
+
 @pytest.mark.parametrize("mode", ["fp32", "fp64"])
 def test_fast_modes_generic_kernel_large_labels(mode):
     # labels above 2^16 take the generic (non-tiled) separable kernel; the
