@@ -1224,6 +1224,9 @@ __global__ void __launch_bounds__(256) k_flip_bfs(FlipCtx c, const int* __restri
     __shared__ BfsShared S;
     __shared__ int last;
     const int n = *qcount;
+    // nothing queued: no search can add to the fallback queue, which the last
+    // launch that searched left empty, so every workgroup leaves at once
+    if (n == 0) return;
     for (int qi = blockIdx.x; qi < n; qi += gridDim.x) {
         const int k = queue[qi];
         const int flags = c.qflags[k];
