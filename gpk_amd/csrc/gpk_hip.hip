@@ -12,6 +12,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -282,6 +283,19 @@ struct Engine final : EngineBase {
     DevBuf Pa, Pb, Plo, Phi, pstate, Pg, pdeps, cmp_cnt, pkey, pvfull;
     bool mask_fresh = false;    // mask matches the current labels
     int flip_batch = 4, pair_batch = 4;   // rounds launched before the first host check
+    // visited cells after which a wavefront / workgroup connectivity search
+    // hands its candidate to the next larger search: the capacities, unless
+    // GPK_FLIP_SEARCH_CAPS="wave,workgroup" lowers them (tests use it to run
+    // the larger searches on small grids)
+    int wave_cap = WB_HMAX, wg_cap = BFS_HMAX;
+    void read_search_caps() {
+        const char* e = std::getenv("GPK_FLIP_SEARCH_CAPS");
+        int a = 0, b = 0;
+        if (e && std::sscanf(e, "%d,%d", &a, &b) == 2) {
+            wave_cap = std::max(1, std::min(a, WB_HMAX));
+            wg_cap = std::max(1, std::min(b, BFS_HMAX));
+        }
+    }
     // fallback connectivity searches take tag = ++tag_ctr (at most two per
     // candidate) and mark cells with (tag << 3) | colour; tag_bound bounds the
     // counter from the host so the marks are cleared before it reaches 2^27
@@ -408,6 +422,7 @@ struct Engine final : EngineBase {
         scal.alloc(sizeof(Scalars)); dsum.alloc(((n + 1023) / 1024 + (n + 1048575) / 1048576 + 16) * 8);
         tagctr.alloc(16); minmax.alloc(16); noise_amp.alloc(16);
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&hs), sizeof(Scalars), hipHostMallocDefault));
+        read_search_caps();
         std::memset(hs, 0, sizeof(Scalars));
         // weights
         std::vector<R> w0(W.w0.begin(), W.w0.end()), w1(W.w1.begin(), W.w1.end()),
@@ -1194,6 +1209,7 @@ struct Engine final : EngineBase {
             fc.pos_new = pos_new.as<int>(); fc.count = count.as<int>();
             fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
             fc.qflags = qflags.as<int>(); fc.s = ds(); fc.slab = (m + 7) / 8;
+            fc.wave_cap = wave_cap; fc.wg_cap = wg_cap;
             reserve_tags(2LL * m);
             int round = 0;
             // one wavefront per queued search (four per workgroup, four 40 KB
@@ -1492,6 +1508,7 @@ struct Engine final : EngineBase {
         fc.Ps = Ps.as<unsigned long long>(); fc.pos_old = pos_old.as<int>(); fc.pos_new = pos_new.as<int>();
         fc.count = count.as<int>(); fc.queue[0] = queue0.as<int>(); fc.queue[1] = queue1.as<int>();
         fc.qflags = qflags.as<int>(); fc.s = ds(); fc.slab = 1;
+        fc.wave_cap = wave_cap; fc.wg_cap = wg_cap;
         reserve_tags(2);
         k_flip_round<<<1, 64, 0, stream>>>(fc, 0);
         k_flip_bfs<<<1, 256, 0, stream>>>(fc, queue0.as<int>(), &ds()->qcount[0][0], fbq.as<int>(), mark.as<int>(),

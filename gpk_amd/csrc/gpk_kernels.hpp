@@ -541,6 +541,7 @@ struct FlipCtx {
     const int* count;          // label counts at the start of the flip pass
     int* queue[2]; int* qflags;
     int slab;                  // candidates per band slab: queue segment x holds slab x's entries
+    int wave_cap, wg_cap;      // visited cells before a wavefront / workgroup search hands over
     Scalars* s;
 };
 
@@ -889,7 +890,7 @@ __device__ __forceinline__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c,
                     if (prev == ~0ull) {
                         const int ins = atomicAdd(&S.inserted, 1);
                         const int pos = atomicAdd(&S.fcount[nxt], 1);
-                        if (ins >= BFS_HMAX || pos >= BFS_FCAP) {
+                        if (ins >= c.wg_cap || pos >= BFS_FCAP) {
                             S.overflow = 1;
                         } else {
                             S.fr[nxt][pos] = nbc; S.frc[nxt][pos] = uint8_t(col);
@@ -1069,7 +1070,7 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
         inserted += ncount;
         adj |= wave_or_sparse(myadj);
         present = wave_or6(mypres);
-        if (__ballot(ovf) != 0ull || ncount > WB_FCAP || inserted > WB_HMAX) return -1;
+        if (__ballot(ovf) != 0ull || ncount > WB_FCAP || inserted > c.wave_cap) return -1;
         nfront = ncount;
         cur = nxt;
         wave_sync();

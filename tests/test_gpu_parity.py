@@ -181,6 +181,19 @@ def test_anisotropic_growth_2d_labels_and_tolerance():
         assert np.all(np.abs(vo - vg) <= 1e-5 * np.maximum(np.abs(vo).max(), 1e-30)), s
 
 
+@pytest.mark.parametrize("mode", ["strict32", "fp64"])
+def test_flip_searches_through_workgroup_and_device_memory(mode, monkeypatch):
+    # wavefront searches hand over after 10 visited cells and workgroup
+    # searches after 30, so on this 48^3 map (25 steps) about 550 searches go
+    # to the workgroup search and about 50 on to the device-memory search
+    # (and bounds searches that outgrow the workgroup wait for their exact
+    # turn); labels and counters must still equal the oracle's
+    monkeypatch.setenv("GPK_FLIP_SEARCH_CAPS", "10,30")
+    phi, g = O.voronoi(48, 48, 48, 132, 42)
+    o, G = pair((48, 48, 48), _abi.isotropic(1.0, 1.0), 0.0, 1.5, mode, {"phi": phi, "g": g})
+    lockstep(o, G, 25, check_every=5)
+
+
 @pytest.mark.parametrize("dims,mode", [((18, 22, 30), "strict32"), ((18, 22, 30), "fp64"), ((1, 54, 70), "fp32")])
 def test_grain_growth_x_not_multiple_of_4(dims, mode):
     # Nx = 30 / 70: the interface mask takes its one-cell-per-thread form (the
