@@ -556,27 +556,61 @@ __device__ __forceinline__ void decide(FlipCtx& c, int k, int v) {
 // earlier transactions touching label L (the label-ordered prefix of Ps up
 // to pos): count change of the accepted ones, and the undecided ones split
 // into those that would add a cell to L and those that would remove one
+// The earlier candidates on label L are the run of Ps (sorted by label,
+// then index) that ends at pos - 1.  Both walks below read it backwards in
+// steps of 8 entries whose keys, then states and new labels, are loaded
+// together, and handle them in order; an entry before the run (or before
+// index 0) reads as another label, so nothing past the run is used.
+constexpr int PREFIX_STEP = 8;
+__device__ __forceinline__ void prefix_keys(const FlipCtx& c, int q0, unsigned long long key[PREFIX_STEP]) {
+#pragma unroll
+    for (int u = 0; u < PREFIX_STEP; ++u) key[u] = q0 - u >= 0 ? c.Ps[q0 - u] : ~0ull;
+}
+__device__ __forceinline__ bool key_on(unsigned long long key, int32_t L) {
+    return int32_t(uint32_t(key >> 32)) == L;   // ~0 reads as label -1, never a label
+}
+
 struct LabelPrefix { int delta, add, rem; };
 __device__ __forceinline__ LabelPrefix label_prefix(const FlipCtx& c, int pos, int32_t L) {
     LabelPrefix r{0, 0, 0};
-    for (int q = pos - 1; q >= 0; --q) {
-        const unsigned long long key = c.Ps[q];
-        if (int32_t(uint32_t(key >> 32)) != L) break;
-        const int j = int(uint32_t(key));
-        const int sj = ld_acq(c.state + j);
-        const bool adds = c.Cnew[j] == L;
-        if (sj == ACCEPT) r.delta += adds ? 1 : -1;
-        else if (sj != REJECT) { if (adds) ++r.add; else ++r.rem; }
+    for (int q0 = pos - 1; q0 >= 0; q0 -= PREFIX_STEP) {
+        unsigned long long key[PREFIX_STEP];
+        prefix_keys(c, q0, key);
+        int sj[PREFIX_STEP], cn[PREFIX_STEP];
+#pragma unroll
+        for (int u = 0; u < PREFIX_STEP; ++u) {
+            sj[u] = REJECT; cn[u] = -1;
+            if (key_on(key[u], L)) {
+                const int j = int(uint32_t(key[u]));
+                sj[u] = ld_acq(c.state + j);
+                cn[u] = c.Cnew[j];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PREFIX_STEP; ++u) {
+            if (!key_on(key[u], L)) return r;
+            const bool adds = cn[u] == L;
+            if (sj[u] == ACCEPT) r.delta += adds ? 1 : -1;
+            else if (sj[u] != REJECT) { if (adds) ++r.add; else ++r.rem; }
+        }
     }
     return r;
 }
 
 // all earlier transactions touching label L decided?
 __device__ __forceinline__ bool label_prefix_ready(const FlipCtx& c, int pos, int32_t L) {
-    for (int q = pos - 1; q >= 0; --q) {
-        const unsigned long long key = c.Ps[q];
-        if (int32_t(uint32_t(key >> 32)) != L) break;
-        if (undecided(ld_acq(c.state + int(uint32_t(key))))) return false;
+    for (int q0 = pos - 1; q0 >= 0; q0 -= PREFIX_STEP) {
+        unsigned long long key[PREFIX_STEP];
+        prefix_keys(c, q0, key);
+        int sj[PREFIX_STEP];
+#pragma unroll
+        for (int u = 0; u < PREFIX_STEP; ++u)
+            sj[u] = key_on(key[u], L) ? ld_acq(c.state + int(uint32_t(key[u]))) : REJECT;
+#pragma unroll
+        for (int u = 0; u < PREFIX_STEP; ++u) {
+            if (!key_on(key[u], L)) return true;
+            if (undecided(sj[u])) return false;
+        }
     }
     return true;
 }
@@ -616,39 +650,43 @@ __device__ int flip_try(FlipCtx& c, int k, int round, int* prev) {
     const int32_t ol = c.Cold[k], nl = c.Cnew[k];
     int z0, y0, x0;
     decode(d, cell, z0, y0, x0);
-    const int zb = c.dim3 ? -1 : 0, ze = c.dim3 ? 1 : 0;
-    uint32_t occ_o = 0, occ_n = 0;
-    int t = 0;
-    for (int dz = zb; dz <= ze; ++dz) {
-        int z = z0 + dz; if (z < 0) z += d.Nz; else if (z >= d.Nz) z -= d.Nz;
-        for (int dy = -1; dy <= 1; ++dy) {
+    // the label words of the whole 3^d cube (position t = (dz+1)*9 +
+    // (dy+1)*3 + (dx+1); 2-D: (dy+1)*3 + (dx+1)) are loaded together first
+    const int centre = c.dim3 ? 13 : 4;
+    const int npos = c.dim3 ? 27 : 9;
+    int32_t lw[27];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        lw[t] = 0;
+        if (t < npos && t != centre) {
+            const int dz = c.dim3 ? t / 9 - 1 : 0, dy = (t / 3) % 3 - 1, ddx = t % 3 - 1;
+            int z = z0 + dz; if (z < 0) z += d.Nz; else if (z >= d.Nz) z -= d.Nz;
             int y = y0 + dy; if (y < 0) y += d.Ny; else if (y >= d.Ny) y -= d.Ny;
-            for (int ddx = -1; ddx <= 1; ++ddx, ++t) {
-                int x = x0 + ddx; if (x < 0) x += d.Nx; else if (x >= d.Nx) x -= d.Nx;
-                const int nb = lin(d, z, y, x);
-                if (nb == cell && dz == 0 && dy == 0 && ddx == 0) continue;
-                int32_t lab = c.phi[nb];
-                if (lab < 0) {
-                    const int j = ~lab;
-                    const int jo = c.Cold[j];
-                    lab = jo;
-                    if (j < k) {
-                        const int jn = c.Cnew[j];
-                        const int sj = ld_acq(c.state + j);
-                        if (jo == ol || jo == nl || jn == ol || jn == nl) {
-                            if (undecided(sj)) return 0;   // not ready
-                        }
-                        if (sj == ACCEPT) lab = jn;
-                    }
-                }
-                if (lab == ol) occ_o |= 1u << t;
-                if (lab == nl) occ_n |= 1u << t;
-            }
+            int x = x0 + ddx; if (x < 0) x += d.Nx; else if (x >= d.Nx) x -= d.Nx;
+            lw[t] = c.phi[lin(d, z, y, x)];
         }
     }
-    const int centre = c.dim3 ? 13 : 4;
-    occ_o &= ~(1u << centre);
-    occ_n &= ~(1u << centre);
+    uint32_t occ_o = 0, occ_n = 0;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        if (t >= npos || t == centre) continue;
+        int32_t lab = lw[t];
+        if (lab < 0) {
+            const int j = ~lab;
+            const int jo = c.Cold[j];
+            lab = jo;
+            if (j < k) {
+                const int jn = c.Cnew[j];
+                const int sj = ld_acq(c.state + j);
+                if (jo == ol || jo == nl || jn == ol || jn == nl) {
+                    if (undecided(sj)) return 0;   // not ready
+                }
+                if (sj == ACCEPT) lab = jn;
+            }
+        }
+        if (lab == ol) occ_o |= 1u << t;
+        if (lab == nl) occ_n |= 1u << t;
+    }
     const int oo = topo_count(occ_o, c.dim3 != 0);
     const int no = topo_count(occ_n, c.dim3 != 0);
     if (no == 0) return decide_claim(c, k, sk, REJECT);
