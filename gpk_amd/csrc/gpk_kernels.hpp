@@ -1013,10 +1013,12 @@ __device__ __forceinline__ int bfs_connected_wave(WaveBfs& S, const FlipCtx& c, 
     // neighbour met twice, on grids of side 2, is one cell of two colours)
     int f[6];
     faces(d, cell, f);
+    int32_t fl[6];
+    face_view_labels(c, mode, k, L, true, nface, f, fl);
     int seed[6], nb = 0;
     unsigned long long adj = 0ull;
     for (int q = 0; q < nface; ++q) {
-        if (view_label_b(mode, f[q], k, L, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
+        if (fl[q] != L) continue;
         seed[nb++] = f[q];
     }
     if (nb <= 1) return 1;
