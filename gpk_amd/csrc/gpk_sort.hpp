@@ -31,9 +31,17 @@ __global__ void __launch_bounds__(RS_T) k_rs_count(int n, const unsigned long lo
     __syncthreads();
     const unsigned long long mask = (1ull << bits) - 1ull;
     const int base = blockIdx.x * RS_CHUNK;
+    // the thread's keys loaded together, then counted
+    unsigned long long kr[RS_ITEMS];
+#pragma unroll
     for (int it = 0; it < RS_ITEMS; ++it) {
         const int i = base + it * RS_T + int(threadIdx.x);
-        if (i < n) atomicAdd(&h[int((keys[i] >> shift) & mask)], 1);
+        kr[it] = i < n ? keys[i] : 0ull;
+    }
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int i = base + it * RS_T + int(threadIdx.x);
+        if (i < n) atomicAdd(&h[int((kr[it] >> shift) & mask)], 1);
     }
     __syncthreads();
     cnt[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
@@ -51,6 +59,16 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(int n, const unsigned long 
     const unsigned long long lt = (1ull << lane) - 1ull;
     const unsigned long long mask = (1ull << bits) - 1ull;
     const int base = blockIdx.x * RS_CHUNK;
+    // the thread's keys and payloads loaded together before the ranked passes
+    unsigned long long kr[RS_ITEMS];
+    int vr[RS_ITEMS];
+#pragma unroll
+    for (int it = 0; it < RS_ITEMS; ++it) {
+        const int i = base + it * RS_T + int(threadIdx.x);
+        kr[it] = i < n ? kin[i] : 0ull;
+        vr[it] = (vin && i < n) ? vin[i] : 0;
+    }
+#pragma unroll
     for (int it = 0; it < RS_ITEMS; ++it) {
         if (base + it * RS_T >= n) break;          // uniform over the block
         for (int q = threadIdx.x; q < NW * RS_BINS; q += RS_T) (&wbase[0][0])[q] = 0;
@@ -60,7 +78,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(int n, const unsigned long 
         unsigned long long key = 0ull;
         int d = 0;
         if (valid) {
-            key = kin[i];
+            key = kr[it];
             d = int((key >> shift) & mask);
         }
         unsigned long long peers = __ballot(valid);
@@ -86,7 +104,7 @@ __global__ void __launch_bounds__(RS_T) k_rs_scatter(int n, const unsigned long 
         if (valid) {
             const int pos = wbase[w][d] + rank;
             kout[pos] = key;
-            if (vin) vout[pos] = vin[i];
+            if (vin) vout[pos] = vr[it];
         }
         __syncthreads();
     }
