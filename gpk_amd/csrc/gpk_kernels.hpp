@@ -134,6 +134,29 @@ __global__ void __launch_bounds__(CMP_T) k_cmp_count(long long n, const uint8_t*
 // one block: bcnt (counts) -> exclusive offsets, *total = sum
 __global__ void __launch_bounds__(1024) k_cmp_scan(int nb, int* __restrict__ bcnt, int* __restrict__ total) {
     __shared__ int wsum[16], tot;
+    constexpr int PER = 32;
+    if (nb <= 1024 * PER) {
+        // each thread a contiguous run of at most PER counts, loaded together:
+        // one block scan of the run sums, then the run's prefixes written
+        const int per = (nb + 1023) / 1024;
+        const int b0 = int(threadIdx.x) * per;
+        int v[PER];
+        int sum = 0;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            v[u] = (u < per && b0 + u < nb) ? bcnt[b0 + u] : 0;
+            sum += v[u];
+        }
+        const int incl = block_incl_scan<1024>(sum, wsum, &tot);
+        int run = incl - sum;
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            if (u < per && b0 + u < nb) bcnt[b0 + u] = run;
+            run += v[u];
+        }
+        if (threadIdx.x == 0) *total = tot;
+        return;
+    }
     int running = 0;
     for (int b0 = 0; b0 < nb; b0 += 1024) {
         const int b = b0 + threadIdx.x;
