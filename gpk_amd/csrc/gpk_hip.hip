@@ -1331,7 +1331,7 @@ struct Engine final : EngineBase {
                 int rounds = 0, batch = pair_batch;
                 while (true) {
                     for (int r = 0; r < batch; ++r, ++rounds)
-                        k_pair_round<R><<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>());
+                        k_pair_round<R><<<nblk(npairs, 4), 256, 0, stream>>>(pc, pdeps.as<int>(), rounds + 1);
                     pull_scalars();
                     if (hs->undecided <= 0) break;
                     if (rounds > 4 * (npairs + 8))

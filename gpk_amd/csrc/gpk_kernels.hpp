@@ -1514,11 +1514,21 @@ __device__ __forceinline__ void st_agent(double* p, double v) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A pair's state word is UNDEC, or its verdict | (round tag << 4), the tag
+// being the round (from 1) that took it.  A verdict taken in the current
+// round counts as undecided: a pair reads only verdicts of earlier rounds,
+// whose orientations (Pg) earlier launches wrote, so no release / acquire
+// between the waves of one launch is needed (the launch boundary orders them).
+__device__ __forceinline__ bool pair_settled(int st, int tag) { return st != UNDEC && (st >> 4) < tag; }
+__device__ __forceinline__ int pair_verdict(int st) { return st & 15; }
+
 template <class R>
-__device__ __forceinline__ void pair_view(const PairCtx& c, int cell, int q, int32_t& lab, R& gv) {
+__device__ __forceinline__ void pair_view(const PairCtx& c, int cell, int q, int tag, int32_t& lab, R& gv) {
     const int p = c.pid[cell];
     const R* g = static_cast<const R*>(c.g);
-    if (p >= 0 && p < q && __hip_atomic_load(c.pstate + p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == ACCEPT) {
+    int sp = UNDEC;
+    if (p >= 0 && p < q) sp = ld_acq(c.pstate + p);
+    if (pair_settled(sp, tag) && pair_verdict(sp) == ACCEPT) {
         const R* Pg = static_cast<const R*>(c.Pg);
         if (c.Pa[p] == cell) { lab = c.Phi[p]; gv = ld_agent(Pg + 2 * p); }
         else { lab = c.Plo[p]; gv = ld_agent(Pg + 2 * p + 1); }
@@ -1578,7 +1588,7 @@ __global__ void __launch_bounds__(256) k_pair_deps(PairCtx c, int* __restrict__ 
 // orientation of every cell of the 5^d boxes around a and b in parallel; the
 // certificates' box masks come from ballots, and lane 0 takes the decision.
 template <class R>
-__global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __restrict__ deps) {
+__global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __restrict__ deps, int tag) {
     constexpr int WPB = 4;                       // wavefronts (pairs) per block
     __shared__ int32_t blab[WPB][2][128];        // versioned labels, box A / box B
     __shared__ R bg[WPB][2][128];                // versioned orientations
@@ -1599,7 +1609,7 @@ __global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __rest
     const int nd = deps[q * PAIR_DEPS];
     bool wait = false;
     if (nd >= 0) {
-        if (lane < nd) wait = ld_acq(c.pstate + deps[q * PAIR_DEPS + 1 + lane]) == UNDEC;
+        if (lane < nd) wait = !pair_settled(ld_acq(c.pstate + deps[q * PAIR_DEPS + 1 + lane]), tag);
     } else {
         for (int k = lane; k < 2 * nbox; k += 64) {
             const int side = k >= nbox ? 1 : 0;
@@ -1610,7 +1620,7 @@ __global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __rest
             const int p = c.pid[cell];
             if (p >= 0 && p < q) {
                 const int32_t pl = c.Plo[p], ph = c.Phi[p];
-                if ((pl == lo || pl == hi || ph == lo || ph == hi) && ld_acq(c.pstate + p) == UNDEC)
+                if ((pl == lo || pl == hi || ph == lo || ph == hi) && !pair_settled(ld_acq(c.pstate + p), tag))
                     wait = true;
             }
         }
@@ -1625,7 +1635,7 @@ __global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __rest
             const int cell = lin(d, wrapi(z0 + lz - 2, d.Nz), wrapi(y0 + ly - 2, d.Ny), wrapi(x0 + lx - 2, d.Nx));
             int32_t l;
             R gv;
-            pair_view<R>(c, cell, q, l, gv);
+            pair_view<R>(c, cell, q, tag, l, gv);
             blab[w][side][r] = l;
             bg[w][side][r] = gv;
             bcell[w][side][r] = cell;
@@ -1683,7 +1693,7 @@ __global__ void __launch_bounds__(256) k_pair_round(PairCtx c, const int* __rest
             st_agent(Pg + 2 * q + 1, gb);
         }
     }
-    __hip_atomic_store(c.pstate + q, verdict, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    st_rel(c.pstate + q, verdict | (tag << 4));
     if (verdict == ACCEPT) atomicAdd(&c.s->exchanges, 2);
     atomicSub(&c.s->undecided, 1);
 }
@@ -1700,7 +1710,7 @@ __global__ void k_pair_apply(int np, const int* __restrict__ Pa, const int* __re
     if (q >= np) return;
     const int a = Pa[q], b = Pb[q];
     pid[a] = -1; pid[b] = -1;
-    if (pstate[q] != ACCEPT) return;
+    if (pair_verdict(pstate[q]) != ACCEPT) return;
     phi[a] = Phi[q]; phi[b] = Plo[q];
     g[a] = Pg[2 * q]; g[b] = Pg[2 * q + 1];
     alpha[a] = R(1); alpha[b] = R(1);
@@ -1722,7 +1732,7 @@ __global__ void k_mark_dirty(Dims d, int dim3, int r, int n, int ncell, const in
         const long long e = t / box;
         const int o = int(t - e * box);
         const int k = int(e / ncell);
-        if (st && st[k] != ACCEPT) continue;
+        if (st && (st[k] & 15) != ACCEPT) continue;   // pair states carry a round tag
         const int cell = (e - (long long)k * ncell) == 0 ? c0[k] : c1[k];
         int z0, y0, x0;
         decode(d, cell, z0, y0, x0);
@@ -1743,7 +1753,7 @@ __global__ void k_mark_dirty_rows(Dims d, int dim3, int r, int n, int ncell, con
         const int e = t / rows;
         const int o = t - e * rows;
         const int k = e / ncell;
-        if (st && st[k] != ACCEPT) continue;
+        if (st && (st[k] & 15) != ACCEPT) continue;   // pair states carry a round tag
         const int cell = e - k * ncell == 0 ? c0[k] : c1[k];
         int z0, y0, x0;
         decode(d, cell, z0, y0, x0);
