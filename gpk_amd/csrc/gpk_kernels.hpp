@@ -883,9 +883,11 @@ __device__ __forceinline__ int bfs_connected_lds(BfsShared& S, const FlipCtx& c,
     if (threadIdx.x == 0) {
         int f[6];
         faces(d, cell, f);
+        int32_t fl[6];
+        face_view_labels(c, mode, k, L, true, nface, f, fl);
         int nb = 0;
         for (int q = 0; q < nface; ++q) {
-            if (view_label_b(mode, f[q], k, L, c.phi, c.state, c.Cold, c.Cnew) != L) continue;
+            if (fl[q] != L) continue;
             const int col = nb++;
             const unsigned long long val = ((unsigned long long)(uint32_t)f[q] << 8) | col;
             uint32_t h = hslot(f[q]);
