@@ -1257,7 +1257,11 @@ struct Engine final : EngineBase {
         set_int(&ds()->exchanges, 0);
         const bool tiled = nlab <= 65536 && remap_r == 3;
         const int nb = band_compact(bandB, tiled);
-        k_copy_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), phi_comp.as<int32_t>());
+        if (N % 4 == 0)
+            k_copy_i32x4<<<gsblk(N / 4, 256), 256, 0, stream>>>(N / 4, reinterpret_cast<const int4*>(phi.p),
+                                                                 reinterpret_cast<int4*>(phi_comp.p));
+        else
+            k_copy_i32<<<gsblk(N, 256), 256, 0, stream>>>(N, phi.as<int32_t>(), phi_comp.as<int32_t>());
         if (rs_reset || !tiled) {
             fill<uint8_t>(rs_valid, N, 0);
             fill<uint8_t>(rs_dirty, N, 0);

@@ -1393,6 +1393,12 @@ __global__ void k_copy_i32(int n, const int32_t* __restrict__ src, int32_t* __re
         dst[i] = src[i];
 }
 
+// the same copy 16 bytes per thread (n a multiple of 4, 16-byte aligned buffers)
+__global__ void k_copy_i32x4(int n4, const int4* __restrict__ src, int4* __restrict__ dst) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 // proposal keys: first pass by gain (descending) then cell, second by
 // (low, high, direction)
 template <class R>
